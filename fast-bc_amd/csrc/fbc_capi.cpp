@@ -1,0 +1,744 @@
+/*
+ * C ABI of the gfx950 betweenness kernels (declared in include/fbc_gpu.h).
+ *
+ * Owns device memory, builds the device CSR layouts (packed {dst, w} arcs,
+ * uint32 row starts, reverse CSR, per-cluster local CSRs) and drives the
+ * persistent kernel in brandes.hip.
+ */
+#include "fbc_gpu.h"
+#include "fbc_internal.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using fbcdev::BrandesParams;
+
+struct fbc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    uint32_t delta_override = 0, slots_override = 0;
+    int cu_count = 0;
+    size_t total_mem = 0;
+    uint8_t *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    uint32_t *d_queue = nullptr, *d_err = nullptr;
+    unsigned long long *d_te = nullptr;
+    double last_ms = 0, last_te = 0;
+};
+
+struct fbc_graph {
+    fbc_ctx *ctx = nullptr;
+    uint32_t n = 0;
+    uint64_t m = 0;
+    uint32_t maxw = 0;
+    double meanw = 0;
+    std::vector<uint32_t> rp, col, w, rrp, rsrc, rw;   /* host copies */
+    uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
+    uint32_t *d_rp = nullptr, *d_rrp = nullptr;
+};
+
+struct fbc_clusters {
+    const fbc_graph *g = nullptr;
+    uint32_t ncl = 0;
+    uint32_t nmax = 0;
+    std::vector<uint32_t> n, bptr, blocal, gid, vcl;
+    std::vector<uint64_t> rpoff, voff, vioff, edges;
+    uint64_t vi_total = 0;
+    uint32_t *d_n = nullptr, *d_gid = nullptr, *d_bptr = nullptr, *d_blocal = nullptr,
+             *d_vcl = nullptr, *d_rp = nullptr, *d_rrp = nullptr;
+    uint64_t *d_rpoff = nullptr, *d_voff = nullptr, *d_vioff = nullptr, *d_doff = nullptr;
+    uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
+};
+
+namespace {
+
+int fail(fbc_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(fbc_ctx *ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, e == hipErrorOutOfMemory ? FBC_E_OOM : FBC_E_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (!count) count = 1;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <typename T>
+hipError_t upload(T **p, const std::vector<T> &h, hipStream_t s)
+{
+    hipError_t e = dalloc(p, h.size());
+    if (e != hipSuccess || h.empty()) return e;
+    e = hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+/* Near-far window: a few mean arc weights. */
+uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
+{
+    if (ctx->delta_override) return ctx->delta_override;
+    double d = g->meanw * 4.0;
+    if (d < 1) d = 1;
+    if (d > 1e9) d = 1e9;
+    return (uint32_t)d;
+}
+
+/* Make room for 'slots' slots of views up to nmax vertices; returns the slot
+ * count actually available. */
+int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out)
+{
+    const uint64_t sb = fbcdev::slot_bytes(nmax);
+    uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)(ctx->cu_count * 8);
+    if (slots > want_slots) slots = want_slots;
+    if (slots < 1) slots = 1;
+    const size_t budget = std::min<size_t>(ctx->total_mem / 3, (size_t)96 << 30);
+    if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
+    const size_t need = (size_t)slots * sb;
+    if (need > ctx->scratch_bytes) {
+        dfree(ctx->scratch);
+        ctx->scratch_bytes = 0;
+        hipError_t e = hipMalloc((void **)&ctx->scratch, need);
+        if (e != hipSuccess) return hip_fail(ctx, e, "scratch allocation");
+        ctx->scratch_bytes = need;
+    }
+    *slots_out = slots;
+    return 0;
+}
+
+void base_params(BrandesParams &p, const fbc_graph *g)
+{
+    std::memset(&p, 0, sizeof(p));
+    p.arcs = g->d_arcs;
+    p.rp = g->d_rp;
+    p.rarcs = g->d_rarcs;
+    p.rrp = g->d_rrp;
+    p.n = g->n;
+    p.nmax = g->n;
+}
+
+/* Launch one persistent run over p.ntasks tasks and wait for it. */
+int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p)
+{
+    if (p.ntasks == 0) {
+        ctx->last_ms = 0;
+        ctx->last_te = 0;
+        return 0;
+    }
+    uint32_t slots = 0;
+    int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots);
+    if (rc) return rc;
+    p.scratch = ctx->scratch;
+    p.slot_bytes = fbcdev::slot_bytes(p.nmax);
+    p.delta = auto_delta(ctx, g);
+    p.queue_head = ctx->d_queue;
+    p.err = ctx->d_err;
+    p.te = ctx->d_te;
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_queue, 0, sizeof(uint32_t), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_te, 0, sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, p, slots, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    uint32_t err = 0;
+    unsigned long long te = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(err), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
+    ctx->last_ms = ms;
+    ctx->last_te = (double)te;
+    if (err & fbcdev::ERR_SIGMA)
+        return fail(ctx, FBC_E_SIGMA_OVERFLOW, "shortest-path count exceeds 2^64-1");
+    if (err & fbcdev::ERR_DIST) return fail(ctx, FBC_E_DIST_OVERFLOW, "path length exceeds 2^32-2");
+    return 0;
+}
+
+bool valid_prec(int prec) { return prec == FBC_PREC_F64 || prec == FBC_PREC_F32C; }
+
+} // namespace
+
+extern "C" {
+
+int fbc_create(int device, fbc_ctx **out)
+{
+    if (!out) return FBC_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return FBC_E_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return FBC_E_HIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return FBC_E_DEVICE;
+    fbc_ctx *ctx = new (std::nothrow) fbc_ctx;
+    if (!ctx) return FBC_E_OOM;
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->total_mem = prop.totalGlobalMem;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        dalloc(&ctx->d_queue, 1) != hipSuccess || dalloc(&ctx->d_err, 1) != hipSuccess ||
+        dalloc(&ctx->d_te, 1) != hipSuccess) {
+        fbc_destroy(ctx);
+        return FBC_E_HIP;
+    }
+    *out = ctx;
+    return 0;
+}
+
+void fbc_destroy(fbc_ctx *ctx)
+{
+    if (!ctx) return;
+    dfree(ctx->scratch);
+    dfree(ctx->d_queue);
+    dfree(ctx->d_err);
+    dfree(ctx->d_te);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *fbc_last_error(const fbc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots)
+{
+    if (!ctx) return FBC_E_ARG;
+    ctx->delta_override = bucket_delta;
+    ctx->slots_override = slots;
+    return 0;
+}
+
+int fbc_graph_upload(fbc_ctx *ctx, uint32_t n, uint64_t m, const uint64_t *row_ptr,
+                     const uint32_t *col, const uint32_t *w, fbc_graph **out)
+{
+    if (!ctx || !out || !row_ptr || (m && (!col || !w))) return fail(ctx, FBC_E_ARG, "null argument");
+    *out = nullptr;
+    if (n == 0 || n >= 0xFFFFFFF0u) return fail(ctx, FBC_E_ARG, "vertex count out of range");
+    if (m >= 0xFFFFFFF0ull) return fail(ctx, FBC_E_UNSUPPORTED, "more than 2^32-16 arcs");
+    if (row_ptr[0] != 0 || row_ptr[n] != m) return fail(ctx, FBC_E_ARG, "row_ptr must run from 0 to m");
+    fbc_graph *g = new (std::nothrow) fbc_graph;
+    if (!g) return fail(ctx, FBC_E_OOM, "host allocation");
+    g->ctx = ctx;
+    g->n = n;
+    g->m = m;
+    try {
+        g->rp.resize((size_t)n + 1);
+        g->col.assign(col, col + m);
+        g->w.assign(w, w + m);
+        double wsum = 0;
+        for (uint32_t v = 0; v <= n; ++v) {
+            g->rp[v] = (uint32_t)row_ptr[v];
+            if (v < n && row_ptr[v + 1] < row_ptr[v]) {
+                delete g;
+                return fail(ctx, FBC_E_ARG, "row_ptr is not monotone");
+            }
+        }
+        for (uint64_t e = 0; e < m; ++e) {
+            if (col[e] >= n || w[e] == 0) {
+                delete g;
+                return fail(ctx, FBC_E_ARG, "arc target out of range or zero weight");
+            }
+            g->maxw = std::max(g->maxw, w[e]);
+            wsum += w[e];
+        }
+        g->meanw = m ? wsum / (double)m : 1.0;
+        /* reverse CSR, rows in ascending source order */
+        g->rrp.assign((size_t)n + 1, 0);
+        for (uint64_t e = 0; e < m; ++e) g->rrp[col[e] + 1]++;
+        for (uint32_t v = 0; v < n; ++v) g->rrp[v + 1] += g->rrp[v];
+        g->rsrc.resize(m);
+        g->rw.resize(m);
+        std::vector<uint32_t> fill(g->rrp.begin(), g->rrp.end() - 1);
+        for (uint32_t u = 0; u < n; ++u)
+            for (uint32_t e = g->rp[u]; e < g->rp[u + 1]; ++e) {
+                uint32_t pos = fill[col[e]]++;
+                g->rsrc[pos] = u;
+                g->rw[pos] = w[e];
+            }
+        std::vector<uint2> arcs(m), rarcs(m);
+        for (uint64_t e = 0; e < m; ++e) {
+            arcs[e] = make_uint2(col[e], w[e]);
+            rarcs[e] = make_uint2(g->rsrc[e], g->rw[e]);
+        }
+        hipError_t e;
+        if ((e = upload(&g->d_arcs, arcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_rarcs, rarcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_rp, g->rp, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_rrp, g->rrp, ctx->stream)) != hipSuccess) {
+            fbc_graph_free(g);
+            return hip_fail(ctx, e, "graph upload");
+        }
+    } catch (const std::bad_alloc &) {
+        delete g;
+        return fail(ctx, FBC_E_OOM, "host allocation");
+    }
+    *out = g;
+    return 0;
+}
+
+void fbc_graph_free(fbc_graph *g)
+{
+    if (!g) return;
+    dfree(g->d_arcs);
+    dfree(g->d_rarcs);
+    dfree(g->d_rp);
+    dfree(g->d_rrp);
+    delete g;
+}
+
+/* Output-row modes (sssp / dep): sources in chunks that fit a device budget. */
+static int row_mode(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, const uint32_t *src,
+                    uint32_t nsrc, uint64_t *dist, uint64_t *sigma, double *dep)
+{
+    for (uint32_t i = 0; i < nsrc; ++i)
+        if (src[i] >= g->n) return fail(ctx, FBC_E_ARG, "source out of range");
+    const size_t row_bytes = (size_t)g->n * (mode == fbcdev::MODE_SSSP ? 16 : 8);
+    const size_t budget = (size_t)4 << 30;
+    uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsrc, budget / row_bytes));
+    uint64_t *d_dist = nullptr, *d_sigma = nullptr;
+    double *d_dep = nullptr;
+    uint32_t *d_src = nullptr;
+    int rc = 0;
+    double ms = 0, te = 0;
+    hipError_t e = dalloc(&d_src, chunk);
+    if (e == hipSuccess) {
+        if (mode == fbcdev::MODE_SSSP) {
+            e = dalloc(&d_dist, (size_t)chunk * g->n);
+            if (e == hipSuccess) e = dalloc(&d_sigma, (size_t)chunk * g->n);
+        } else {
+            e = dalloc(&d_dep, (size_t)chunk * g->n);
+        }
+    }
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "output allocation");
+    for (uint32_t s0 = 0; !rc && s0 < nsrc; s0 += chunk) {
+        const uint32_t k = std::min(chunk, nsrc - s0);
+        BrandesParams p;
+        base_params(p, g);
+        p.ntasks = k;
+        p.task_src = d_src;
+        p.out_dist = d_dist;
+        p.out_sigma = d_sigma;
+        p.out_dep = d_dep;
+        if ((e = hipMemcpyAsync(d_src, src + s0, k * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+            (d_dep && (e = hipMemsetAsync(d_dep, 0, (size_t)k * g->n * sizeof(double), ctx->stream)) != hipSuccess)) {
+            rc = hip_fail(ctx, e, "chunk setup");
+            break;
+        }
+        rc = run(ctx, g, mode, prec, true, p);
+        if (rc) break;
+        ms += ctx->last_ms;
+        te += ctx->last_te;
+        if (mode == fbcdev::MODE_SSSP) {
+            e = hipMemcpy(dist + (size_t)s0 * g->n, d_dist, (size_t)k * g->n * 8, hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                e = hipMemcpy(sigma + (size_t)s0 * g->n, d_sigma, (size_t)k * g->n * 8, hipMemcpyDeviceToHost);
+        } else {
+            e = hipMemcpy(dep + (size_t)s0 * g->n, d_dep, (size_t)k * g->n * 8, hipMemcpyDeviceToHost);
+        }
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "result download");
+    }
+    dfree(d_src);
+    dfree(d_dist);
+    dfree(d_sigma);
+    dfree(d_dep);
+    ctx->last_ms = ms;
+    ctx->last_te = te;
+    return rc;
+}
+
+int fbc_sssp_sigma(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_t nsrc,
+                   uint64_t *dist, uint64_t *sigma)
+{
+    if (!ctx || !g || (nsrc && (!src || !dist || !sigma))) return fail(ctx, FBC_E_ARG, "null argument");
+    return row_mode(ctx, g, fbcdev::MODE_SSSP, 0, src, nsrc, dist, sigma, nullptr);
+}
+
+int fbc_dep_batch(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_t nsrc, int prec,
+                  double *dep)
+{
+    if (!ctx || !g || (nsrc && (!src || !dep))) return fail(ctx, FBC_E_ARG, "null argument");
+    if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    return row_mode(ctx, g, fbcdev::MODE_DEP, prec, src, nsrc, nullptr, nullptr, dep);
+}
+
+int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec, double *dep)
+{
+    return fbc_dep_batch(ctx, g, &src, 1, prec, dep);
+}
+
+int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc)
+{
+    if (!ctx || !g || !bc) return fail(ctx, FBC_E_ARG, "null argument");
+    if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    std::vector<uint32_t> src(g->n);
+    std::iota(src.begin(), src.end(), 0u);
+    uint32_t *d_src = nullptr;
+    double *d_bc = nullptr;
+    hipError_t e;
+    if ((e = upload(&d_src, src, ctx->stream)) != hipSuccess || (e = dalloc(&d_bc, g->n)) != hipSuccess ||
+        (e = hipMemsetAsync(d_bc, 0, (size_t)g->n * sizeof(double), ctx->stream)) != hipSuccess) {
+        dfree(d_src);
+        dfree(d_bc);
+        return hip_fail(ctx, e, "exact setup");
+    }
+    BrandesParams p;
+    base_params(p, g);
+    p.ntasks = g->n;
+    p.task_src = d_src;
+    p.bc = d_bc;
+    int rc = run(ctx, g, fbcdev::MODE_EXACT, prec, false, p);
+    if (!rc && (e = hipMemcpy(bc, d_bc, (size_t)g->n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(ctx, e, "result download");
+    dfree(d_src);
+    dfree(d_bc);
+    return rc;
+}
+
+int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const uint32_t *cluster_ptr,
+                        const uint32_t *cluster_verts, fbc_clusters **out, uint32_t *border_ptr)
+{
+    if (!ctx || !g || !out || !cluster_ptr || (ncl && !cluster_verts)) return fail(ctx, FBC_E_ARG, "null argument");
+    *out = nullptr;
+    const uint32_t n = g->n;
+    const uint64_t total = cluster_ptr[ncl];
+    if (cluster_ptr[0] != 0 || total > n) return fail(ctx, FBC_E_ARG, "bad cluster_ptr");
+    fbc_clusters *cl = new (std::nothrow) fbc_clusters;
+    if (!cl) return fail(ctx, FBC_E_OOM, "host allocation");
+    cl->g = g;
+    cl->ncl = ncl;
+    try {
+        std::vector<int32_t> owner(n, -1);
+        std::vector<uint32_t> lid(n, 0);
+        for (uint32_t c = 0; c < ncl; ++c) {
+            if (cluster_ptr[c + 1] < cluster_ptr[c]) throw std::invalid_argument("bad cluster_ptr");
+            for (uint32_t i = cluster_ptr[c]; i < cluster_ptr[c + 1]; ++i) {
+                const uint32_t v = cluster_verts[i];
+                if (v >= n || owner[v] != -1) throw std::invalid_argument("cluster vertex out of range or repeated");
+                if (i > cluster_ptr[c] && v <= cluster_verts[i - 1])
+                    throw std::invalid_argument("cluster vertex lists must be ascending");
+                owner[v] = (int32_t)c;
+                lid[v] = i - cluster_ptr[c];
+            }
+        }
+        cl->n.resize(ncl);
+        cl->rpoff.resize(ncl);
+        cl->voff.resize(ncl);
+        cl->vioff.resize(ncl);
+        cl->edges.resize(ncl);
+        cl->bptr.assign((size_t)ncl + 1, 0);
+        cl->gid.assign(cluster_verts, cluster_verts + total);
+        cl->vcl.resize(total);
+        std::vector<uint32_t> rp, rrp;
+        std::vector<uint2> arcs, rarcs;
+        rp.reserve(total + ncl);
+        rrp.reserve(total + ncl);
+        uint64_t vi = 0;
+        for (uint32_t c = 0; c < ncl; ++c) {
+            const uint32_t v0 = cluster_ptr[c], nc = cluster_ptr[c + 1] - v0;
+            cl->n[c] = nc;
+            cl->nmax = std::max(cl->nmax, nc);
+            cl->voff[c] = v0;
+            cl->rpoff[c] = rp.size();
+            cl->vioff[c] = vi;
+            uint64_t internal = 0;
+            for (uint32_t i = 0; i < nc; ++i) {
+                const uint32_t v = cluster_verts[v0 + i];
+                cl->vcl[v0 + i] = c;
+                rp.push_back((uint32_t)arcs.size());
+                rrp.push_back((uint32_t)rarcs.size());
+                bool border = false;
+                for (uint32_t e = g->rp[v]; e < g->rp[v + 1]; ++e) {
+                    const uint32_t t = g->col[e];
+                    if (owner[t] == (int32_t)c) {
+                        arcs.push_back(make_uint2(lid[t], g->w[e]));
+                        ++internal;
+                    } else {
+                        border = true;
+                    }
+                }
+                for (uint32_t e = g->rrp[v]; e < g->rrp[v + 1]; ++e) {
+                    const uint32_t s = g->rsrc[e];
+                    if (owner[s] == (int32_t)c) rarcs.push_back(make_uint2(lid[s], g->rw[e]));
+                    else border = true;
+                }
+                if (border) cl->blocal.push_back(i);
+            }
+            rp.push_back((uint32_t)arcs.size());
+            rrp.push_back((uint32_t)rarcs.size());
+            cl->edges[c] = internal;
+            cl->bptr[c + 1] = (uint32_t)cl->blocal.size();
+            vi += (uint64_t)nc * (cl->bptr[c + 1] - cl->bptr[c]);
+        }
+        cl->vi_total = vi;
+        hipError_t e;
+        if ((e = upload(&cl->d_n, cl->n, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_gid, cl->gid, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_bptr, cl->bptr, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_blocal, cl->blocal, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_vcl, cl->vcl, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_rp, rp, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_rrp, rrp, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_arcs, arcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_rarcs, rarcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_rpoff, cl->rpoff, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_voff, cl->voff, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_vioff, cl->vioff, ctx->stream)) != hipSuccess ||
+            (e = dalloc(&cl->d_doff, ncl)) != hipSuccess) {
+            fbc_clusters_free(cl);
+            return hip_fail(ctx, e, "cluster upload");
+        }
+    } catch (const std::bad_alloc &) {
+        fbc_clusters_free(cl);
+        return fail(ctx, FBC_E_OOM, "host allocation");
+    } catch (const std::invalid_argument &ex) {
+        fbc_clusters_free(cl);
+        return fail(ctx, FBC_E_ARG, ex.what());
+    }
+    if (border_ptr) std::copy(cl->bptr.begin(), cl->bptr.end(), border_ptr);
+    *out = cl;
+    return 0;
+}
+
+int fbc_clusters_borders(const fbc_clusters *cl, uint32_t *borders)
+{
+    if (!cl || !borders) return FBC_E_ARG;
+    for (uint32_t c = 0; c < cl->ncl; ++c)
+        for (uint32_t j = cl->bptr[c]; j < cl->bptr[c + 1]; ++j)
+            borders[j] = cl->gid[cl->voff[c] + cl->blocal[j]];
+    return 0;
+}
+
+int64_t fbc_clusters_edges(const fbc_clusters *cl, uint32_t c)
+{
+    if (!cl || c >= cl->ncl) return FBC_E_ARG;
+    return (int64_t)cl->edges[c];
+}
+
+void fbc_clusters_free(fbc_clusters *cl)
+{
+    if (!cl) return;
+    dfree(cl->d_n);
+    dfree(cl->d_gid);
+    dfree(cl->d_bptr);
+    dfree(cl->d_blocal);
+    dfree(cl->d_vcl);
+    dfree(cl->d_rp);
+    dfree(cl->d_rrp);
+    dfree(cl->d_arcs);
+    dfree(cl->d_rarcs);
+    dfree(cl->d_rpoff);
+    dfree(cl->d_voff);
+    dfree(cl->d_vioff);
+    dfree(cl->d_doff);
+    delete cl;
+}
+
+int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uint64_t *vi_len,
+                     uint64_t *vi_cnt)
+{
+    if (!ctx || !cl || !intra_bc || (cl->vi_total && (!vi_len || !vi_cnt)))
+        return fail(ctx, FBC_E_ARG, "null argument");
+    const fbc_graph *g = cl->g;
+    double *d_intra = nullptr, *d_dmat = nullptr;
+    uint64_t *d_vil = nullptr, *d_vic = nullptr;
+    uint32_t *d_tsrc = nullptr, *d_taux = nullptr;
+    hipError_t e;
+    int rc = 0;
+    /* dependency matrices (n_c x n_c doubles): all bordered clusters at once
+     * when they fit in 8 GiB, else chunks; never less than the largest one */
+    const size_t dbudget = ((size_t)8 << 30) / sizeof(double);
+    size_t dcap = 0, dall = 0;
+    for (uint32_t c = 0; c < cl->ncl; ++c)
+        if (cl->bptr[c + 1] > cl->bptr[c]) {
+            dcap = std::max(dcap, (size_t)cl->n[c] * cl->n[c]);
+            dall += (size_t)cl->n[c] * cl->n[c];
+        }
+    dcap = std::max(dcap, std::min(dall, dbudget));
+    if ((e = dalloc(&d_intra, g->n)) != hipSuccess || (e = dalloc(&d_dmat, dcap)) != hipSuccess ||
+        (e = dalloc(&d_vil, cl->vi_total)) != hipSuccess || (e = dalloc(&d_vic, cl->vi_total)) != hipSuccess ||
+        (e = hipMemcpy(d_intra, intra_bc, (size_t)g->n * 8, hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = hip_fail(ctx, e, "cluster eval setup");
+    }
+    double ms = 0, te = 0;
+    uint32_t c0 = 0;
+    while (!rc && c0 < cl->ncl) {
+        /* a chunk of consecutive clusters whose dependency matrices fit */
+        std::vector<uint64_t> doff(cl->ncl, ~uint64_t(0));
+        std::vector<uint32_t> order;
+        size_t used = 0;
+        uint32_t c1 = c0;
+        while (c1 < cl->ncl) {
+            const size_t sz = (size_t)cl->n[c1] * cl->n[c1];
+            const bool bordered = cl->bptr[c1 + 1] > cl->bptr[c1];
+            if (bordered && used + sz > dcap && !order.empty()) break;
+            if (bordered) {
+                doff[c1] = used;
+                used += sz;
+                order.push_back(c1);
+            }
+            ++c1;
+        }
+        /* largest clusters first */
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return (double)cl->n[a] * cl->edges[a] > (double)cl->n[b] * cl->edges[b];
+        });
+        std::vector<uint32_t> tsrc, taux;
+        for (uint32_t c : order)
+            for (uint32_t s = 0; s < cl->n[c]; ++s) {
+                tsrc.push_back(s);
+                taux.push_back(c);
+            }
+        if (!tsrc.empty()) {
+            dfree(d_tsrc);
+            dfree(d_taux);
+            if ((e = upload(&d_tsrc, tsrc, ctx->stream)) != hipSuccess ||
+                (e = upload(&d_taux, taux, ctx->stream)) != hipSuccess ||
+                (e = hipMemcpy(cl->d_doff, doff.data(), doff.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemsetAsync(d_dmat, 0, used * 8, ctx->stream)) != hipSuccess) {
+                rc = hip_fail(ctx, e, "cluster chunk setup");
+                break;
+            }
+            BrandesParams p;
+            base_params(p, g);
+            p.arcs = cl->d_arcs;
+            p.rarcs = cl->d_rarcs;
+            p.rp = cl->d_rp;
+            p.rrp = cl->d_rrp;
+            p.nmax = cl->nmax;
+            p.cl_n = cl->d_n;
+            p.cl_rpoff = cl->d_rpoff;
+            p.cl_voff = cl->d_voff;
+            p.cl_gid = cl->d_gid;
+            p.cl_bptr = cl->d_bptr;
+            p.cl_blocal = cl->d_blocal;
+            p.cl_vioff = cl->d_vioff;
+            p.cl_doff = cl->d_doff;
+            p.ntasks = (uint32_t)tsrc.size();
+            p.task_src = d_tsrc;
+            p.task_aux = d_taux;
+            p.dmat = d_dmat;
+            p.vi_len = d_vil;
+            p.vi_cnt = d_vic;
+            rc = run(ctx, g, fbcdev::MODE_CLUSTER, 0, true, p);
+            if (rc) break;
+            ms += ctx->last_ms;
+            te += ctx->last_te;
+            /* ordered per-vertex sums over the chunk's vertex range (borderless
+             * clusters carry an all-ones matrix offset and are skipped) */
+            const uint64_t vb = cl->voff[c0];
+            const uint64_t ve = cl->voff[c1 - 1] + cl->n[c1 - 1];
+            e = fbcdev::launch_intra_reduce(p, vb, ve - vb, cl->d_vcl, d_intra, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "intra reduce");
+        }
+        c0 = c1;
+    }
+    if (!rc) {
+        if ((e = hipMemcpy(intra_bc, d_intra, (size_t)g->n * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (cl->vi_total && (e = hipMemcpy(vi_len, d_vil, cl->vi_total * 8, hipMemcpyDeviceToHost)) != hipSuccess) ||
+            (cl->vi_total && (e = hipMemcpy(vi_cnt, d_vic, cl->vi_total * 8, hipMemcpyDeviceToHost)) != hipSuccess))
+            rc = hip_fail(ctx, e, "result download");
+    }
+    dfree(d_intra);
+    dfree(d_dmat);
+    dfree(d_vil);
+    dfree(d_vic);
+    dfree(d_tsrc);
+    dfree(d_taux);
+    ctx->last_ms = ms;
+    ctx->last_te = te;
+    return rc;
+}
+
+int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot, const uint32_t *weight,
+                    const uint32_t *pivot_cluster, uint32_t npiv, const double *intra_bc, int prec,
+                    double *bc)
+{
+    if (!ctx || !cl || !bc || !intra_bc || (npiv && (!pivot || !weight || !pivot_cluster)))
+        return fail(ctx, FBC_E_ARG, "null argument");
+    if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    const fbc_graph *g = cl->g;
+    for (uint32_t i = 0; i < npiv; ++i)
+        if (pivot[i] >= g->n || pivot_cluster[i] >= cl->ncl) return fail(ctx, FBC_E_ARG, "pivot out of range");
+    /* pivot weights grouped by cluster, in the given order */
+    std::vector<uint32_t> cptr((size_t)cl->ncl + 1, 0), pw(npiv);
+    for (uint32_t i = 0; i < npiv; ++i) cptr[pivot_cluster[i] + 1]++;
+    for (uint32_t c = 0; c < cl->ncl; ++c) cptr[c + 1] += cptr[c];
+    {
+        std::vector<uint32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (uint32_t i = 0; i < npiv; ++i) pw[fill[pivot_cluster[i]]++] = weight[i];
+    }
+    std::vector<uint32_t> tsrc(pivot, pivot + npiv), taux(weight, weight + npiv);
+    double *d_bc = nullptr, *d_intra = nullptr;
+    uint32_t *d_tsrc = nullptr, *d_taux = nullptr, *d_cptr = nullptr, *d_pw = nullptr;
+    hipError_t e;
+    int rc = 0;
+    if ((e = dalloc(&d_bc, g->n)) != hipSuccess || (e = dalloc(&d_intra, g->n)) != hipSuccess ||
+        (e = upload(&d_tsrc, tsrc, ctx->stream)) != hipSuccess || (e = upload(&d_taux, taux, ctx->stream)) != hipSuccess ||
+        (e = upload(&d_cptr, cptr, ctx->stream)) != hipSuccess || (e = upload(&d_pw, pw, ctx->stream)) != hipSuccess ||
+        (e = hipMemcpy(d_bc, bc, (size_t)g->n * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d_intra, intra_bc, (size_t)g->n * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        rc = hip_fail(ctx, e, "pivot setup");
+    if (!rc) {
+        BrandesParams p;
+        base_params(p, g);
+        p.ntasks = npiv;
+        p.task_src = d_tsrc;
+        p.task_aux = d_taux;
+        p.bc = d_bc;
+        rc = run(ctx, g, fbcdev::MODE_PIVOT, prec, false, p);
+    }
+    if (!rc) {
+        e = fbcdev::launch_pivot_subtract(cl->gid.size(), cl->d_gid, cl->d_vcl, d_cptr, d_pw, d_intra, d_bc,
+                                          ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(bc, d_bc, (size_t)g->n * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "pivot finish");
+    }
+    dfree(d_bc);
+    dfree(d_intra);
+    dfree(d_tsrc);
+    dfree(d_taux);
+    dfree(d_cptr);
+    dfree(d_pw);
+    return rc;
+}
+
+int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edges)
+{
+    if (!ctx) return FBC_E_ARG;
+    if (kernel_ms) *kernel_ms = ctx->last_ms;
+    if (traversed_edges) *traversed_edges = ctx->last_te;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,90 @@
+/* Shared between the HIP kernels and the C-ABI layer (not a public header). */
+#ifndef FBC_INTERNAL_H
+#define FBC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbcdev {
+
+constexpr uint32_t DINF = 0xFFFFFFFFu;     /* unreached */
+constexpr int WG = 256;                    /* threads per source (4 wavefronts) */
+
+enum Mode : int {
+    MODE_SSSP = 0,      /* write dist / sigma rows (test hook) */
+    MODE_DEP = 1,       /* write dependency rows */
+    MODE_EXACT = 2,     /* bc[v] += dep(v) */
+    MODE_PIVOT = 3,     /* bc[v] += w * dep(v) */
+    MODE_CLUSTER = 4    /* dependency matrix rows + VertexInfo rows */
+};
+
+enum ErrBits : uint32_t {
+    ERR_DIST = 1u,          /* path length beyond 2^32-2 */
+    ERR_SIGMA = 2u,         /* sigma beyond 2^64-1 */
+    ERR_SIGMA_INEXACT = 4u  /* sigma >= 2^53: the reference's FP64 sigma is inexact */
+};
+
+/* Everything a launch needs, passed by value. */
+struct BrandesParams {
+    /* graph view used when cl_n == nullptr */
+    const uint2 *arcs;        /* {dst, w}, rows sorted by dst */
+    const uint32_t *rp;       /* row starts, n+1 */
+    const uint2 *rarcs;       /* reverse arcs {src, w} */
+    const uint32_t *rrp;
+    uint32_t n;
+
+    /* cluster views: concatenated local CSRs (row starts are absolute) */
+    const uint32_t *cl_n;     /* vertices per cluster */
+    const uint64_t *cl_rpoff; /* offset of the cluster's row-start block (n_c + 1 entries) */
+    const uint64_t *cl_voff;  /* offset of the cluster's vertex block */
+    const uint32_t *cl_gid;   /* local -> global id, by cl_voff */
+    const uint32_t *cl_bptr;  /* border offsets (ncl + 1) */
+    const uint32_t *cl_blocal;/* local ids of the borders, ascending */
+    const uint64_t *cl_vioff; /* VertexInfo block offset per cluster */
+    const uint64_t *cl_doff;  /* dependency-matrix offset per cluster (this chunk) */
+
+    /* tasks: source (local id in cluster mode), aux = cluster index or weight */
+    uint32_t ntasks;
+    const uint32_t *task_src;
+    const uint32_t *task_aux;
+    uint32_t *queue_head;
+
+    /* per-workgroup scratch */
+    uint8_t *scratch;
+    uint64_t slot_bytes;
+    uint32_t nmax;
+    uint32_t delta;           /* near-far window */
+
+    /* outputs */
+    double *bc;
+    double *out_dep;
+    uint64_t *out_dist;
+    uint64_t *out_sigma;
+    double *dmat;
+    uint64_t *vi_len;
+    uint64_t *vi_cnt;
+    uint32_t *err;
+    unsigned long long *te;   /* traversed-edge counter */
+};
+
+/* Bytes of one slot for views of at most nmax vertices. */
+uint64_t slot_bytes(uint32_t nmax);
+
+/* Launch the persistent kernel (grid = slots workgroups) on 'stream'. */
+hipError_t launch_brandes(int mode, int prec, bool ordered, const BrandesParams &p,
+                          uint32_t slots, hipStream_t stream);
+
+/* intra_bc[gid(v)] += sum over s (ascending) of dmat[doff_c + s*n_c + v] for
+ * the concatenated cluster vertices [vbegin, vbegin + vcount) whose cluster
+ * has a matrix in this chunk (doff_c != ~0). */
+hipError_t launch_intra_reduce(const BrandesParams &p, uint64_t vbegin, uint64_t vcount,
+                               const uint32_t *v_cluster, double *intra, hipStream_t stream);
+
+/* bc[v] -= intra[v] * w_p for every pivot p of v's cluster, pivots in order. */
+hipError_t launch_pivot_subtract(uint64_t nv, const uint32_t *v_gid, const uint32_t *v_cluster,
+                                 const uint32_t *cpiv_ptr, const uint32_t *piv_w,
+                                 const double *intra, double *bc, hipStream_t stream);
+
+} // namespace fbcdev
+
+#endif

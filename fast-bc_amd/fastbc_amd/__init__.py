@@ -1,0 +1,333 @@
+"""ctypes bindings of the MI355X betweenness library (include/fbc_gpu.h).
+
+The library is built in-tree (fast-bc_amd/lib/libfastbc_amd.so, by
+`make -C fast-bc_amd` or `__graft_entry__.build()`).  Nothing here falls back
+to a CPU implementation: a missing library raises at load time and every
+failing call raises FbcError with the library's message.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libfastbc_amd.so")
+GEN_PATH = os.path.join(PKG_DIR, "lib", "libfbcgen.so")
+FBC_BIN = os.path.join(PKG_DIR, "bin", "fbc")
+
+PREC_F64 = 0
+PREC_F32C = 1
+
+# Every symbol include/fbc_gpu.h declares.
+EXPORTS = (
+    "fbc_create", "fbc_destroy", "fbc_last_error", "fbc_set_tuning", "fbc_graph_upload",
+    "fbc_graph_free", "fbc_sssp_sigma", "fbc_single_source_dep", "fbc_dep_batch",
+    "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
+    "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
+)
+
+ERRORS = {
+    -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",
+    -5: "FBC_E_SIGMA_OVERFLOW", -6: "FBC_E_DIST_OVERFLOW", -7: "FBC_E_UNSUPPORTED",
+}
+
+
+class FbcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_gen = None
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load libfastbc_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C fast-bc_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.fbc_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        L.fbc_destroy.argtypes = [_vp]
+        L.fbc_last_error.argtypes = [_vp]
+        L.fbc_last_error.restype = ctypes.c_char_p
+        L.fbc_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.fbc_graph_upload.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint64, _u64p, _u32p, _u32p,
+                                       ctypes.POINTER(_vp)]
+        L.fbc_graph_free.argtypes = [_vp]
+        L.fbc_sssp_sigma.argtypes = [_vp, _vp, _u32p, ctypes.c_uint32, _u64p, _u64p]
+        L.fbc_single_source_dep.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _f64p]
+        L.fbc_dep_batch.argtypes = [_vp, _vp, _u32p, ctypes.c_uint32, ctypes.c_int, _f64p]
+        L.fbc_exact_bc.argtypes = [_vp, _vp, ctypes.c_int, _f64p]
+        L.fbc_clusters_create.argtypes = [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp),
+                                          _u32p]
+        L.fbc_clusters_borders.argtypes = [_vp, _u32p]
+        L.fbc_clusters_edges.argtypes = [_vp, ctypes.c_uint32]
+        L.fbc_clusters_edges.restype = ctypes.c_int64
+        L.fbc_clusters_free.argtypes = [_vp]
+        L.fbc_cluster_eval.argtypes = [_vp, _vp, _f64p, _u64p, _u64p]
+        L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
+                                      ctypes.c_int, _f64p]
+        L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Context:
+    """A device context (fbc_create)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = lib().fbc_create(device, ctypes.byref(h))
+        if rc:
+            raise FbcError(rc, "fbc_create failed (a gfx950 device is required)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fbc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc:
+            raise FbcError(rc, lib().fbc_last_error(self.h).decode())
+
+    def set_tuning(self, bucket_delta=0, slots=0):
+        self.check(lib().fbc_set_tuning(self.h, bucket_delta, slots))
+
+    def last_stats(self):
+        ms, te = ctypes.c_double(), ctypes.c_double()
+        lib().fbc_last_stats(self.h, ctypes.byref(ms), ctypes.byref(te))
+        return ms.value, te.value
+
+    def graph(self, n, row_ptr, col, w):
+        return Graph(self, n, row_ptr, col, w)
+
+
+class Graph:
+    """A device-resident CSR graph (fbc_graph_upload)."""
+
+    def __init__(self, ctx, n, row_ptr, col, w):
+        self.ctx = ctx
+        self.n = int(n)
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        col = _u32(col)
+        w = _u32(w)
+        self.m = int(rp[-1])
+        h = _vp()
+        ctx.check(lib().fbc_graph_upload(ctx.h, self.n, self.m, rp, col, w, ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_arcs(cls, ctx, src, dst, w, n=None):
+        """Arcs must be sorted by (src, dst) without repeats."""
+        src = _u32(src)
+        n = int(src.max()) + 1 if n is None else n
+        rp = np.zeros(n + 1, np.uint64)
+        np.add.at(rp, src.astype(np.int64) + 1, 1)
+        rp = np.cumsum(rp).astype(np.uint64)
+        return cls(ctx, n, rp, dst, w)
+
+    def close(self):
+        if self.h:
+            lib().fbc_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sssp_sigma(self, sources):
+        s = _u32(sources)
+        d = np.zeros(len(s) * self.n, np.uint64)
+        sg = np.zeros(len(s) * self.n, np.uint64)
+        self.ctx.check(lib().fbc_sssp_sigma(self.ctx.h, self.h, s, len(s), d, sg))
+        return d.reshape(len(s), self.n), sg.reshape(len(s), self.n)
+
+    def single_source_dep(self, src, prec=PREC_F64):
+        dep = np.zeros(self.n)
+        self.ctx.check(lib().fbc_single_source_dep(self.ctx.h, self.h, int(src), prec, dep))
+        return dep
+
+    def dep_batch(self, sources, prec=PREC_F64):
+        s = _u32(sources)
+        dep = np.zeros(len(s) * self.n)
+        self.ctx.check(lib().fbc_dep_batch(self.ctx.h, self.h, s, len(s), prec, dep))
+        return dep.reshape(len(s), self.n)
+
+    def exact_bc(self, prec=PREC_F64):
+        bc = np.zeros(self.n)
+        self.ctx.check(lib().fbc_exact_bc(self.ctx.h, self.h, prec, bc))
+        return bc
+
+    def clusters(self, comm_of=None, communities=None):
+        return Clusters(self, comm_of=comm_of, communities=communities)
+
+
+class Clusters:
+    """Cluster views (fbc_clusters_create) of a vertex partition."""
+
+    def __init__(self, graph, comm_of=None, communities=None):
+        self.graph = graph
+        ctx = graph.ctx
+        if communities is None:
+            comm_of = np.asarray(comm_of, np.int64)
+            ncl = int(comm_of.max()) + 1 if len(comm_of) else 0
+            order = np.argsort(comm_of, kind="stable")
+            counts = np.bincount(comm_of, minlength=ncl)
+            ptr = np.zeros(ncl + 1, np.uint32)
+            ptr[1:] = np.cumsum(counts)
+            verts = _u32(order)
+        else:
+            ncl = len(communities)
+            ptr = np.zeros(ncl + 1, np.uint32)
+            ptr[1:] = np.cumsum([len(c) for c in communities])
+            verts = _u32(np.concatenate([np.asarray(c) for c in communities]) if ncl else [])
+        self.ncl = ncl
+        self.ptr = ptr
+        self.verts = verts
+        self.border_ptr = np.zeros(ncl + 1, np.uint32)
+        h = _vp()
+        ctx.check(lib().fbc_clusters_create(ctx.h, graph.h, ncl, ptr, verts, ctypes.byref(h),
+                                             self.border_ptr))
+        self.h = h
+        self.borders_flat = np.zeros(max(int(self.border_ptr[-1]), 1), np.uint32)
+        ctx.check(lib().fbc_clusters_borders(self.h, self.borders_flat))
+        nb = np.diff(self.border_ptr.astype(np.int64))
+        nc = np.diff(ptr.astype(np.int64))
+        self.vi_off = np.zeros(ncl + 1, np.int64)
+        self.vi_off[1:] = np.cumsum(nc * nb)
+
+    def close(self):
+        if self.h:
+            lib().fbc_clusters_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def members(self, c):
+        return self.verts[self.ptr[c]:self.ptr[c + 1]]
+
+    def borders(self, c):
+        return self.borders_flat[self.border_ptr[c]:self.border_ptr[c + 1]]
+
+    def edges(self, c):
+        return int(lib().fbc_clusters_edges(self.h, c))
+
+    def evaluate(self, intra_bc=None):
+        """Returns (intra_bc, vi_len, vi_cnt) with VertexInfo rows flattened."""
+        n = self.graph.n
+        bc = np.zeros(n) if intra_bc is None else np.ascontiguousarray(intra_bc, np.float64).copy()
+        tot = max(int(self.vi_off[-1]), 1)
+        L = np.zeros(tot, np.uint64)
+        C = np.zeros(tot, np.uint64)
+        self.graph.ctx.check(lib().fbc_cluster_eval(self.graph.ctx.h, self.h, bc, L, C))
+        return bc, L, C
+
+    def vi_rows(self, flat, c):
+        nb = int(self.border_ptr[c + 1] - self.border_ptr[c])
+        nc = int(self.ptr[c + 1] - self.ptr[c])
+        return flat[self.vi_off[c]:self.vi_off[c] + nc * nb].reshape(nc, nb)
+
+    def pivot_phase(self, pivots, weights, pivot_cluster, intra_bc, bc, prec=PREC_F64):
+        out = np.ascontiguousarray(bc, np.float64).copy()
+        p = _u32(pivots)
+        self.graph.ctx.check(lib().fbc_pivot_phase(
+            self.graph.ctx.h, self.h, p, _u32(weights), _u32(pivot_cluster), len(p),
+            np.ascontiguousarray(intra_bc, np.float64), prec, out))
+        return out
+
+
+# ---- generators ---------------------------------------------------------------
+
+class _Arcs(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint64),
+                ("src", ctypes.POINTER(ctypes.c_uint32)), ("dst", ctypes.POINTER(ctypes.c_uint32)),
+                ("w", ctypes.POINTER(ctypes.c_uint32))]
+
+
+def _genlib():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise RuntimeError(f"{GEN_PATH} is missing: build it with `make -C fast-bc_amd`")
+        G = ctypes.CDLL(GEN_PATH)
+        G.fbcgen_config.argtypes = [ctypes.c_int, ctypes.POINTER(_Arcs)]
+        G.fbcgen_grid.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                  ctypes.c_int, ctypes.POINTER(_Arcs)]
+        G.fbcgen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_Arcs)]
+        G.fbcgen_write.argtypes = [ctypes.POINTER(_Arcs), ctypes.c_char_p]
+        G.fbcgen_free.argtypes = [ctypes.POINTER(_Arcs)]
+        _gen = G
+    return _gen
+
+
+def _take(a):
+    m = a.m
+    out = (np.ctypeslib.as_array(a.src, (m,)).copy(), np.ctypeslib.as_array(a.dst, (m,)).copy(),
+           np.ctypeslib.as_array(a.w, (m,)).copy())
+    n = a.n
+    _genlib().fbcgen_free(ctypes.byref(a))
+    return n, out
+
+
+def generate_config(cfg):
+    """(n, (src, dst, w)) of benchmark workload cfg 0..4 (see gen/fbc_gen.h)."""
+    a = _Arcs()
+    rc = _genlib().fbcgen_config(cfg, ctypes.byref(a))
+    if rc:
+        raise ValueError(f"fbcgen_config({cfg}) failed: {rc}")
+    return _take(a)
+
+
+def generate_grid(rows, cols, wmax, seed, road=True):
+    a = _Arcs()
+    rc = _genlib().fbcgen_grid(rows, cols, wmax, seed, int(road), ctypes.byref(a))
+    if rc:
+        raise ValueError(f"fbcgen_grid failed: {rc}")
+    return _take(a)
+
+
+def generate_rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, wmax=255, seed=4):
+    s = _Arcs()
+    rc = _genlib().fbcgen_rmat(scale, edge_factor, a, b, c, wmax, seed, ctypes.byref(s))
+    if rc:
+        raise ValueError(f"fbcgen_rmat failed: {rc}")
+    return _take(s)
+
+
+def write_edge_list(n, arcs, path):
+    src, dst, w = (np.ascontiguousarray(x, np.uint32) for x in arcs)
+    with open(path, "w") as f:
+        np.savetxt(f, np.stack([src, dst, w], 1), fmt="%d")
+
+
+def csr_from_arcs(n, src, dst, w):
+    """Row pointers for arcs sorted by (src, dst)."""
+    rp = np.zeros(n + 1, np.uint64)
+    np.add.at(rp, np.asarray(src, np.int64) + 1, 1)
+    return np.cumsum(rp).astype(np.uint64), _u32(dst), _u32(w)

@@ -1,0 +1,143 @@
+/*
+ * fbc_gpu.h -- C ABI of the MI355X (gfx950) betweenness-centrality kernels.
+ *
+ * The reference (15831944/fast-bc) is a header-only C++17 template library
+ * with no ABI; its hot path is reached through four virtual interfaces.  Each
+ * entry point below replaces the body of one of them and is called by the C++
+ * host layer in fast-bc_amd/host (which keeps the reference's class names):
+ *
+ *   ISSBrandesBC::singleSourceBrandes     brandes/ISSBrandesBC.h:26-28
+ *       -> fbc_single_source_dep / fbc_dep_batch
+ *   IBrandesBC::computeBC (ExactBrandesBC) brandes/IBrandesBC.h:24, brandes/ExactBrandesBC.h:46-95
+ *       -> fbc_exact_bc
+ *   IClusterEvaluator::evaluateCluster     brandes/IClusterEvaluator.h:33-36
+ *       -> fbc_clusters_create + fbc_cluster_eval (all clusters in one launch)
+ *   ClusteredBrandeBC pivot loop           brandes/ClusteredBrandesBC.h:141-166
+ *       -> fbc_pivot_phase (all pivots in one launch)
+ *
+ * Conventions
+ *   - Return 0 on success or a negative FBC_E_* status; fbc_last_error()
+ *     gives a message.  No C++ exceptions cross this boundary.
+ *   - Host buffers are owned by the caller; device memory by the context.
+ *   - A context is not thread-safe; use one per host thread.
+ *   - Calls block until their results are in the caller's buffers.
+ *   - Graphs are CSR with 0-based uint32 vertex ids and positive integer
+ *     uint32 weights (the reference accepts any positive double weight; the
+ *     GPU path keeps path lengths exact and therefore needs integers).
+ *   - Distances and shortest-path counts are exact integers: path lengths in
+ *     uint32 inside the kernels (checked), sigma in uint64 with an overflow
+ *     check (FBC_E_SIGMA_OVERFLOW).
+ */
+#ifndef FBC_GPU_H
+#define FBC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FBC_OK = 0,
+    FBC_E_ARG = -1,            /* bad argument */
+    FBC_E_HIP = -2,            /* HIP runtime error */
+    FBC_E_OOM = -3,            /* device or host allocation failed */
+    FBC_E_DEVICE = -4,         /* device is not a gfx950 (MI355X) */
+    FBC_E_SIGMA_OVERFLOW = -5, /* a shortest-path count exceeded 2^64-1 */
+    FBC_E_DIST_OVERFLOW = -6,  /* a path length exceeded 2^32-2 */
+    FBC_E_UNSUPPORTED = -7
+};
+
+/* Arithmetic of the dependency (delta) sums. */
+enum {
+    FBC_PREC_F64 = 0,          /* FP64, the reference's operation order per vertex */
+    FBC_PREC_F32C = 1          /* FP32 pairs with two-sum / two-product compensation */
+};
+
+typedef struct fbc_ctx fbc_ctx;
+typedef struct fbc_graph fbc_graph;
+typedef struct fbc_clusters fbc_clusters;
+
+/* Context on HIP device 'device' (must be gfx950).  Replaces nothing in the
+ * reference (its computers hold no device state). */
+int fbc_create(int device, fbc_ctx **out);
+void fbc_destroy(fbc_ctx *ctx);
+const char *fbc_last_error(const fbc_ctx *ctx);
+
+/* Tuning knobs (0 = automatic).  bucket_delta is the delta-stepping window;
+ * slots bounds the sources in flight (one workgroup each). */
+int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
+
+/* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
+ * Replaces the device-side use of DirectedWeightedGraph's adjacency maps
+ * (DirectedWeightedGraph.h:57-58, forwardStar :111-115). */
+int fbc_graph_upload(fbc_ctx *ctx, uint32_t n, uint64_t m, const uint64_t *row_ptr,
+                     const uint32_t *col, const uint32_t *w, fbc_graph **out);
+void fbc_graph_free(fbc_graph *g);
+
+/* Test hook: distances and sigma from nsrc sources, row-major [nsrc x n].
+ * Unreached vertices get dist = UINT64_MAX and sigma = 0.
+ * Follows the forward pass of DijkstraSSBrandesBC::_dijkstra_SSSP
+ * (brandes/DijkstraSSBrandesBC.h:83-147). */
+int fbc_sssp_sigma(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_t nsrc,
+                   uint64_t *dist, uint64_t *sigma);
+
+/* ISSBrandesBC::singleSourceBrandes (brandes/DijkstraSSBrandesBC.h:45-81):
+ * dep[n] = dependency of every vertex on paths from src (0 for src and for
+ * unreached vertices).  One source per call: the plain form. */
+int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec,
+                          double *dep);
+
+/* The same for nsrc sources in one launch; dep is [nsrc x n] row-major. */
+int fbc_dep_batch(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_t nsrc,
+                  int prec, double *dep);
+
+/* ExactBrandesBC::computeBC (brandes/ExactBrandesBC.h:46-95): bc[n] = sum over
+ * all sources s of dep_s(v), v != s.  All sources in one persistent launch. */
+int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc);
+
+/* Clusters: ncl vertex sets given as cluster_ptr[ncl+1] / cluster_verts (each
+ * set ascending, sets disjoint).  Builds SubGraph views (SubGraph.h:59-148):
+ * border = vertex with an arc leaving or entering its set; the intra-cluster
+ * forward stars.  border_ptr[ncl+1] receives the border counts as offsets. */
+int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl,
+                        const uint32_t *cluster_ptr, const uint32_t *cluster_verts,
+                        fbc_clusters **out, uint32_t *border_ptr);
+/* Ascending border ids of every cluster, concatenated per border_ptr. */
+int fbc_clusters_borders(const fbc_clusters *cl, uint32_t *borders);
+/* Internal arc count of cluster c (SubGraph::edges, SubGraph.h:106). */
+int64_t fbc_clusters_edges(const fbc_clusters *cl, uint32_t c);
+void fbc_clusters_free(fbc_clusters *cl);
+
+/* DijkstraClusterEvaluator::evaluateCluster (DijkstraClusterEvaluator.h:50-100)
+ * for every cluster with at least one border, all sources in one launch.
+ *   intra_bc[n]   += intra-cluster betweenness (FP64, summed over the sources in
+ *                    vertex order, as the reference's one-thread team does);
+ *   vi_len / vi_cnt  rows of VertexInfo: for cluster c, source row r (r-th
+ *                    vertex of the set) and border j the entry sits at
+ *                    vi_off[c] + r * B_c + j, vi_off[c] = sum over c' < c of
+ *                    n_c' * B_c'.  len = path length to the border (0 when
+ *                    unreached), cnt = sigma (uint64; the reference keeps
+ *                    (int)sigma).
+ * Borderless clusters are skipped (FASTBC_BRANDES_CLUSTERED_IGNORE_UNCONNECTED). */
+int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc,
+                     uint64_t *vi_len, uint64_t *vi_cnt);
+
+/* The pivot loop of ClusteredBrandeBC::computeBC (ClusteredBrandesBC.h:141-166):
+ *   bc[v] += w_p * dep_p(v)                for every v,
+ *   bc[v] -= w_p * intra_bc[v]             for v in the pivot's cluster,
+ * over npiv pivots (pivot id, weight, cluster index into cl) in one launch.
+ * The dependency sums use 'prec'; the per-vertex reduction is FP64. */
+int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
+                    const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
+                    const double *intra_bc, int prec, double *bc);
+
+/* Kernel time of the last launch of the entry points above, in milliseconds,
+ * and the number of forward arcs it examined (traversed edges). */
+int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,193 @@
+"""Parity of the HIP path (called through the C ABI) against the CPU oracle.
+
+Bars: distances and sigma bit-exact; FP64 dependencies in the reference's
+order bit-exact; sums whose order differs from the reference's (OpenMP
+reductions over sources) within 1e-9 relative; the FP32-compensated form
+within 1e-6 relative of FP64.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fastbc_amd as F
+import oracle_lib as O
+from conftest import GOLDEN, read_arcs
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(os.cpu_count() or 1, 16)
+U64MAX = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def both_graphs(ctx, n, src, dst, w):
+    g = F.Graph(ctx, n, *F.csr_from_arcs(n, src, dst, w))
+    og = O.Graph.from_arcs(src, dst, np.asarray(w, np.float64))
+    assert og.n == n
+    return g, og
+
+
+@pytest.fixture(scope="module")
+def dwg9(gpu_ctx):
+    s, d, w = read_arcs(os.path.join(GOLDEN, "dwg9.txt"))
+    order = np.lexsort((d, s))
+    return both_graphs(gpu_ctx, 9, s[order], d[order], w[order])
+
+
+@pytest.fixture(scope="module")
+def cfg0(gpu_ctx):
+    n, (s, d, w) = F.generate_config(0)
+    return both_graphs(gpu_ctx, n, s, d, w)
+
+
+def check_sssp(g, og, sources):
+    dist, sig = g.sssp_sigma(sources)
+    for i, s in enumerate(sources):
+        od, osg, _ = og.sssp(int(s))
+        reached = np.isfinite(od)
+        assert np.array_equal(dist[i][~reached], np.full((~reached).sum(), U64MAX))
+        assert np.array_equal(dist[i][reached], od[reached].astype(np.uint64))
+        assert np.array_equal(sig[i], osg.astype(np.uint64))
+
+
+def test_dwg9_sssp_sigma_dep(dwg9):
+    g, og = dwg9
+    check_sssp(g, og, list(range(9)))
+    for s in range(9):
+        assert np.array_equal(g.single_source_dep(s), og.single_source_dep(s))
+
+
+def test_dwg9_cluster_known_answer(dwg9):
+    """test/brandes/DijkstraClusterEvaluator.cpp:38-74 on cluster {0..4}."""
+    g, _ = dwg9
+    cl = g.clusters(communities=[[0, 1, 2, 3, 4], [5, 6, 7, 8]])
+    assert list(cl.borders(0)) == [3, 4]
+    assert cl.edges(0) == 7
+    bc, L, C = cl.evaluate()
+    # cluster {5..8} is bordered too, so only vertices 0..4 carry the known answer
+    assert list(bc[:5]) == [2.0, 2.0, 0.5, 1.0, 0.0]
+    assert cl.vi_rows(L, 0).tolist() == [[5, 7], [1, 4], [8, 4], [0, 3], [0, 0]]
+    assert cl.vi_rows(C, 0).tolist() == [[1, 2], [1, 1], [1, 1], [1, 1], [0, 1]]
+
+
+def test_cfg0_sssp_sigma_bitexact(cfg0):
+    g, og = cfg0
+    rng = np.random.default_rng(0)
+    check_sssp(g, og, rng.choice(g.n, 48, replace=False))
+
+
+def test_cfg0_dep_bitexact(cfg0):
+    g, og = cfg0
+    srcs = np.random.default_rng(1).choice(g.n, 32, replace=False)
+    dep = g.dep_batch(srcs)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(dep[i], og.single_source_dep(int(s))), f"source {s}"
+
+
+def test_cfg0_exact_bc(cfg0):
+    g, og = cfg0
+    bc = g.exact_bc()
+    ref = og.exact_bc(THREADS)
+    rel = np.abs(bc - ref) / np.maximum(np.abs(ref), 1e-12 * np.abs(ref).max())
+    assert rel.max() <= 1e-9, rel.max()
+    gold = os.path.join(GOLDEN, "cfg0_exact_bc.npy")
+    if os.path.exists(gold):
+        r2 = np.load(gold)
+        assert (np.abs(bc - r2) / np.maximum(np.abs(r2), 1e-12 * np.abs(r2).max())).max() <= 1e-9
+
+
+def test_cfg0_dep_f32c(cfg0):
+    g, _ = cfg0
+    srcs = np.arange(0, g.n, 257)
+    d64 = g.dep_batch(srcs, F.PREC_F64)
+    d32 = g.dep_batch(srcs, F.PREC_F32C)
+    rel = np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)
+    rel[d64 == 0] = np.abs(d32[d64 == 0])
+    assert rel.max() <= 1e-6, rel.max()
+
+
+def test_cfg0_cluster_eval_bitexact(cfg0):
+    g, og = cfg0
+    comm, k = og.louvain([1, 2, 3, 4], 0.01)
+    cl = g.clusters(comm_of=comm)
+    bc, L, C = cl.evaluate()
+    ref = np.zeros(g.n)
+    for c in range(k):
+        verts = cl.members(c)
+        obc, ob, oL, oC = og.cluster_eval(verts)
+        assert np.array_equal(ob, cl.borders(c))
+        ref[verts] = obc[verts]
+        assert np.array_equal(cl.vi_rows(L, c), oL.astype(np.uint64))
+        assert np.array_equal(cl.vi_rows(C, c).astype(np.int64), oC.astype(np.int64))
+    assert np.array_equal(bc, ref)
+
+
+def test_cfg0_pivot_phase(cfg0):
+    g, og = cfg0
+    r = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=0.5, threads=1)
+    cl = g.clusters(comm_of=r["comm"])
+    out = cl.pivot_phase(r["pivots"], r["weights"], r["pivot_cluster"], r["intra"], r["intra"])
+    ref = r["bc"]
+    # magnitude of the summed terms bounds the rounding of any summation order
+    dep = g.dep_batch(r["pivots"])
+    mag = np.abs(r["intra"]) + (np.abs(dep) * r["weights"][:, None]).sum(0)
+    wsum = np.zeros(cl.ncl)
+    np.add.at(wsum, r["pivot_cluster"], r["weights"])
+    mag += wsum[r["comm"]] * np.abs(r["intra"])
+    err = np.abs(out - ref)
+    assert (err <= 1e-9 * np.abs(ref) + 1e-13 * mag).all(), (err / np.maximum(mag, 1)).max()
+
+
+def test_sigma_overflow_detected(gpu_ctx):
+    """Unit-weight bidirected 70x70 grid: sigma to the far corner is
+    C(138, 69) > 2^64."""
+    n, (s, d, w) = F.generate_grid(70, 70, 1, 7, road=False)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    with pytest.raises(F.FbcError) as e:
+        g.sssp_sigma([0])
+    assert e.value.code == -5
+
+
+def test_unit_grid_sigma_closed_form(gpu_ctx):
+    """sigma(0 -> (r, c)) = C(r + c, r) on a unit-weight bidirected grid."""
+    from math import comb
+    R = 30
+    n, (s, d, w) = F.generate_grid(R, R, 1, 7, road=False)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    dist, sig = g.sssp_sigma([0])
+    r, c = np.divmod(np.arange(n), R)
+    assert np.array_equal(dist[0], (r + c).astype(np.uint64))
+    assert all(int(sig[0][v]) == comb(int(r[v] + c[v]), int(r[v])) for v in range(n))
+
+
+def test_rmat_exact_bc(gpu_ctx):
+    n, (s, d, w) = F.generate_rmat(10, 16, seed=11)
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    check_sssp(g, og, [0, 1, 5, n - 1])
+    bc = g.exact_bc()
+    ref = og.exact_bc(THREADS)
+    rel = np.abs(bc - ref) / np.maximum(np.abs(ref), 1e-12 * np.abs(ref).max())
+    assert rel.max() <= 1e-9, rel.max()
+
+
+def test_cfg2_dist_sigma_properties(gpu_ctx):
+    """Full-size road grid (1,048,576 vertices): Bellman optimality of the
+    distances and sigma(v) = sum of sigma over DAG predecessors."""
+    n, (s, d, w) = F.generate_config(2)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    srcs = [0, 523777, n - 1]
+    dist, sig = g.sssp_sigma(srcs)
+    s64, d64, w64 = s.astype(np.int64), d.astype(np.int64), w.astype(np.int64)
+    for i, src in enumerate(srcs):
+        D = dist[i].astype(np.int64)
+        assert D[src] == 0 and (dist[i] != U64MAX).all()
+        assert (D[d64] <= D[s64] + w64).all()
+        tight = D[s64] + w64 == D[d64]
+        has_pred = np.zeros(n, bool)
+        has_pred[d64[tight]] = True
+        has_pred[src] = True
+        assert has_pred.all()
+        acc = np.zeros(n, np.uint64)
+        np.add.at(acc, d64[tight], sig[i][s64[tight]])
+        acc[src] = 1
+        assert np.array_equal(acc, sig[i])
