@@ -136,6 +136,11 @@ struct View {
     uint32_t n;
 };
 
+/* Per-thread work counters, flushed into BrandesParams::stats. */
+struct Counters {
+    unsigned long long c[STAT_COUNT] = {};
+};
+
 struct Shared {
     uint32_t task;
     uint32_t qcnt[2];
@@ -152,8 +157,9 @@ struct Shared {
 __device__ __forceinline__ void relax_arc(const View &V, const Slot &S, Shared &sh,
                                           uint32_t du, uint32_t e, uint32_t thr,
                                           uint32_t nxt, uint32_t win, uint32_t *qnext,
-                                          uint32_t *far, uint32_t *err)
+                                          uint32_t *far, uint32_t *err, Counters &ct)
 {
+    ++ct.c[STAT_RELAX];
     const uint2 a = V.arcs[e];
     const uint64_t nd64 = (uint64_t)du + a.y;
     if (nd64 >= DINF) {
@@ -164,16 +170,20 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot &S, Shared &
     if (nd >= ld_a32(&S.dist[a.x])) return;
     const uint32_t old = atomicMin(&S.dist[a.x], nd);
     if (nd >= old) return;
+    ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
         if (atomicMax(&S.stamp[a.x], nxt) < nxt) qnext[atomicAdd(&sh.qcnt[nxt & 1], 1u)] = a.x;
     } else {
-        if (atomicMax(&S.fstamp[a.x], win) < win) far[atomicAdd(&sh.fcnt, 1u)] = a.x;
+        if (atomicMax(&S.fstamp[a.x], win) < win) {
+            far[atomicAdd(&sh.fcnt, 1u)] = a.x;
+            ++ct.c[STAT_FARPUSH];
+        }
     }
 }
 
 /* Phase 1: near-far delta-stepping.  On return dist[] is final. */
 __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32_t src,
-                                uint32_t delta, uint32_t *err)
+                                uint32_t delta, uint32_t *err, Counters &ct)
 {
     const int tid = threadIdx.x;
     for (uint32_t v = tid; v < V.n; v += WG) {
@@ -201,8 +211,10 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
     for (;;) {
         while (near) {
             const uint32_t nxt = rnd + 1;
+            if (tid == 0) ++ct.c[STAT_ROUNDS];
             for (uint32_t i = tid; i < near; i += WG) {
                 const uint32_t u = qa[i];
+                ++ct.c[STAT_VISITS];
                 const uint32_t du = ld_a32(&S.dist[u]);
                 const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
                 if (e1 - e0 > (uint32_t)HEAVY_DEG) {
@@ -213,7 +225,7 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
                     }
                 }
                 for (uint32_t e = e0; e < e1; ++e)
-                    relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err);
+                    relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err, ct);
             }
             __syncthreads();
             const uint32_t nh = sh.nheavy < (uint32_t)HEAVY_CAP ? sh.nheavy : (uint32_t)HEAVY_CAP;
@@ -223,7 +235,7 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
                     const uint32_t du = ld_a32(&S.dist[u]);
                     const uint32_t e1 = V.rp[u + 1];
                     for (uint32_t e = V.rp[u] + tid; e < e1; e += WG)
-                        relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err);
+                        relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err, ct);
                 }
                 __syncthreads();
             }
@@ -240,6 +252,7 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
         /* window [.., thr) is settled; move on to the nearest far vertex */
         const uint32_t nfar = sh.fcnt;
         if (nfar == 0) break;
+        if (tid == 0) ++ct.c[STAT_WINDOWS];
         if (tid == 0) sh.minv = DINF;
         __syncthreads();
         uint32_t mn = DINF;
@@ -282,7 +295,7 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
 /* Phase 2: DAG in-degrees, then the Kahn sweep with exact sigma.  Returns the
  * number of levels; the level list is S.qa, the level starts S.fa. */
 __device__ uint32_t phase_sigma(const View &V, const Slot &S, Shared &sh, uint32_t src,
-                                uint32_t *err)
+                                uint32_t *err, Counters &ct)
 {
     const int tid = threadIdx.x;
     uint32_t *cnt = S.stamp;          /* stamps are no longer needed */
@@ -363,6 +376,7 @@ __device__ uint32_t phase_sigma(const View &V, const Slot &S, Shared &sh, uint32
         if (tid == 0) sh.nheavy = 0;
         __syncthreads();
     }
+    if (tid == 0) ct.c[STAT_LEVELS] += nlev;
     return nlev;
 }
 
@@ -484,6 +498,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
     Slot S;
     slot_layout(p.nmax, p.scratch + (uint64_t)blockIdx.x * p.slot_bytes, &S);
     if (tid == 0) sh.te = 0;
+    Counters ct;
 
     for (;;) {
         if (tid == 0) sh.task = atomicAdd(p.queue_head, 1u);
@@ -511,8 +526,8 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
         }
         const uint32_t src = p.task_src[task];
 
-        phase_distances(V, S, sh, src, p.delta, p.err);
-        const uint32_t nlev = phase_sigma(V, S, sh, src, p.err);
+        phase_distances(V, S, sh, src, p.delta, p.err, ct);
+        const uint32_t nlev = phase_sigma(V, S, sh, src, p.err, ct);
 
         if (MODE == MODE_SSSP) {
             uint64_t *od = p.out_dist + (uint64_t)task * V.n;
@@ -561,6 +576,9 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
     }
     __syncthreads();
     if (tid == 0 && sh.te) atomicAdd(p.te, sh.te);
+    if (p.stats)
+        for (int i = 0; i < STAT_COUNT; ++i)
+            if (ct.c[i]) atomicAdd(&p.stats[i], ct.c[i]);
 }
 
 template <int MODE, int PREC, bool ORDERED>

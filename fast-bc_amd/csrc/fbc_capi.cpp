@@ -29,8 +29,9 @@ struct fbc_ctx {
     uint8_t *scratch = nullptr;
     size_t scratch_bytes = 0;
     uint32_t *d_queue = nullptr, *d_err = nullptr;
-    unsigned long long *d_te = nullptr;
+    unsigned long long *d_te = nullptr, *d_stats = nullptr;
     double last_ms = 0, last_te = 0;
+    unsigned long long last_counters[fbcdev::STAT_COUNT] = {};
 };
 
 struct fbc_graph {
@@ -162,6 +163,8 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     p.queue_head = ctx->d_queue;
     p.err = ctx->d_err;
     p.te = ctx->d_te;
+    p.stats = ctx->d_stats;
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(ctx->last_counters), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_queue, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_te, 0, sizeof(unsigned long long), ctx->stream));
@@ -175,6 +178,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     unsigned long long te = 0;
     HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(err), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(ctx->last_counters, ctx->d_stats, sizeof(ctx->last_counters), hipMemcpyDeviceToHost));
     ctx->last_ms = ms;
     ctx->last_te = (double)te;
     if (err & fbcdev::ERR_SIGMA)
@@ -206,7 +210,7 @@ int fbc_create(int device, fbc_ctx **out)
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         dalloc(&ctx->d_queue, 1) != hipSuccess || dalloc(&ctx->d_err, 1) != hipSuccess ||
-        dalloc(&ctx->d_te, 1) != hipSuccess) {
+        dalloc(&ctx->d_te, 1) != hipSuccess || dalloc(&ctx->d_stats, fbcdev::STAT_COUNT) != hipSuccess) {
         fbc_destroy(ctx);
         return FBC_E_HIP;
     }
@@ -221,6 +225,7 @@ void fbc_destroy(fbc_ctx *ctx)
     dfree(ctx->d_queue);
     dfree(ctx->d_err);
     dfree(ctx->d_te);
+    dfree(ctx->d_stats);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -739,6 +744,13 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
     if (kernel_ms) *kernel_ms = ctx->last_ms;
     if (traversed_edges) *traversed_edges = ctx->last_te;
     return 0;
+}
+
+int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count)
+{
+    if (!ctx || !counters || count < 0) return FBC_E_ARG;
+    for (int i = 0; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];
+    return count < fbcdev::STAT_COUNT ? count : fbcdev::STAT_COUNT;
 }
 
 } // extern "C"

@@ -65,6 +65,19 @@ struct BrandesParams {
     uint64_t *vi_cnt;
     uint32_t *err;
     unsigned long long *te;   /* traversed-edge counter */
+    unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
+};
+
+/* Work counters of the persistent kernel (fbc_last_stats_ex). */
+enum Stat : int {
+    STAT_ROUNDS = 0,      /* near rounds (workgroup barriers) of the distance phase */
+    STAT_WINDOWS = 1,     /* near-far windows */
+    STAT_VISITS = 2,      /* near-queue vertices expanded */
+    STAT_RELAX = 3,       /* arcs relaxed in the distance phase */
+    STAT_IMPROVE = 4,     /* relaxations that lowered a distance */
+    STAT_FARPUSH = 5,     /* far-list insertions */
+    STAT_LEVELS = 6,      /* levels of the shortest-path DAG */
+    STAT_COUNT = 8
 };
 
 /* Bytes of one slot for views of at most nmax vertices. */

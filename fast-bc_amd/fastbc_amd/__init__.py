@@ -24,7 +24,10 @@ EXPORTS = (
     "fbc_graph_free", "fbc_sssp_sigma", "fbc_single_source_dep", "fbc_dep_batch",
     "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
+    "fbc_last_counters",
 )
+
+COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels")
 
 ERRORS = {
     -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",
@@ -76,6 +79,7 @@ def lib():
         L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
                                       ctypes.c_int, _f64p]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -116,6 +120,11 @@ class Context:
         ms, te = ctypes.c_double(), ctypes.c_double()
         lib().fbc_last_stats(self.h, ctypes.byref(ms), ctypes.byref(te))
         return ms.value, te.value
+
+    def last_counters(self):
+        c = np.zeros(8, np.uint64)
+        lib().fbc_last_counters(self.h, c, 8)
+        return dict(zip(COUNTER_NAMES, (int(x) for x in c)))
 
     def graph(self, n, row_ptr, col, w):
         return Graph(self, n, row_ptr, col, w)
@@ -321,9 +330,13 @@ def generate_rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, wmax=255, seed=4):
 
 
 def write_edge_list(n, arcs, path):
+    """'<src> <dst> <w>' lines (the reference's input format), written in C."""
     src, dst, w = (np.ascontiguousarray(x, np.uint32) for x in arcs)
-    with open(path, "w") as f:
-        np.savetxt(f, np.stack([src, dst, w], 1), fmt="%d")
+    P = ctypes.POINTER(ctypes.c_uint32)
+    a = _Arcs(n, len(src), src.ctypes.data_as(P), dst.ctypes.data_as(P), w.ctypes.data_as(P))
+    rc = _genlib().fbcgen_write(ctypes.byref(a), path.encode())
+    if rc:
+        raise OSError(-rc, f"writing {path}")
 
 
 def csr_from_arcs(n, src, dst, w):

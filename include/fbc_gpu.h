@@ -136,6 +136,12 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
  * and the number of forward arcs it examined (traversed edges). */
 int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edges);
 
+/* Work counters of the last launch (distance-phase rounds, windows, expanded
+ * vertices, relaxations, improvements, far insertions, DAG levels, summed over
+ * its sources).  Fills at most 'count' entries, returns how many it wrote.
+ * Instrumentation only; no counterpart in the reference. */
+int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
+
 #ifdef __cplusplus
 }
 #endif
