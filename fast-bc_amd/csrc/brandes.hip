@@ -36,14 +36,33 @@ constexpr int SUCC_REG = 8;     /* successors kept in registers by the ordered f
 
 /* ---- memory helpers ------------------------------------------------------ */
 
-/* Relaxed agent-scope loads for words other threads update with atomics. */
+/* Per-slot state is touched by one workgroup only, so its atomics and the
+ * loads of words other threads update atomically use workgroup scope: they
+ * stay in the XCD's L2 instead of going to the device-coherent memory side.
+ * Only the shared outputs (bc) and the task queue use device scope. */
 __device__ __forceinline__ uint32_t ld_a32(const uint32_t *p)
 {
-    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ uint64_t ld_a64(const uint64_t *p)
 {
-    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t wg_min(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t wg_max(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t wg_sub(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint64_t wg_add64(uint64_t *p, uint64_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
@@ -168,13 +187,13 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot &S, Shared &
     }
     const uint32_t nd = (uint32_t)nd64;
     if (nd >= ld_a32(&S.dist[a.x])) return;
-    const uint32_t old = atomicMin(&S.dist[a.x], nd);
+    const uint32_t old = wg_min(&S.dist[a.x], nd);
     if (nd >= old) return;
     ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
-        if (atomicMax(&S.stamp[a.x], nxt) < nxt) qnext[atomicAdd(&sh.qcnt[nxt & 1], 1u)] = a.x;
+        if (wg_max(&S.stamp[a.x], nxt) < nxt) qnext[atomicAdd(&sh.qcnt[nxt & 1], 1u)] = a.x;
     } else {
-        if (atomicMax(&S.fstamp[a.x], win) < win) {
+        if (wg_max(&S.fstamp[a.x], win) < win) {
             far[atomicAdd(&sh.fcnt, 1u)] = a.x;
             ++ct.c[STAT_FARPUSH];
         }
@@ -272,9 +291,9 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
             const uint32_t d = ld_a32(&S.dist[w]);
             if (d < thr) continue;            /* settled in an earlier window */
             if (d < nthr) {
-                if (atomicMax(&S.stamp[w], rnd) < rnd) qa[atomicAdd(&sh.qcnt[rnd & 1], 1u)] = w;
+                if (wg_max(&S.stamp[w], rnd) < rnd) qa[atomicAdd(&sh.qcnt[rnd & 1], 1u)] = w;
             } else {
-                if (atomicMax(&S.fstamp[w], nwin) < nwin) fb[atomicAdd(&sh.fcnt2, 1u)] = w;
+                if (wg_max(&S.fstamp[w], nwin) < nwin) fb[atomicAdd(&sh.fcnt2, 1u)] = w;
             }
         }
         __syncthreads();
@@ -343,9 +362,9 @@ __device__ uint32_t phase_sigma(const View &V, const Slot &S, Shared &sh, uint32
             for (uint32_t e = e0; e < e1; ++e) {
                 const uint2 a = V.arcs[e];
                 if ((uint64_t)du + a.y != ld_a32(&S.dist[a.x])) continue;
-                const uint64_t old = atomicAdd((unsigned long long *)&S.sigma[a.x], (unsigned long long)su);
+                const uint64_t old = wg_add64(&S.sigma[a.x], su);
                 if (old + su < old) atomicOr(err, ERR_SIGMA);
-                if (atomicSub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
+                if (wg_sub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
             }
         }
         __syncthreads();
@@ -359,9 +378,9 @@ __device__ uint32_t phase_sigma(const View &V, const Slot &S, Shared &sh, uint32
                 for (uint32_t e = V.rp[u] + tid; e < e1; e += WG) {
                     const uint2 a = V.arcs[e];
                     if ((uint64_t)du + a.y != ld_a32(&S.dist[a.x])) continue;
-                    const uint64_t old = atomicAdd((unsigned long long *)&S.sigma[a.x], (unsigned long long)su);
+                    const uint64_t old = wg_add64(&S.sigma[a.x], su);
                     if (old + su < old) atomicOr(err, ERR_SIGMA);
-                    if (atomicSub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
+                    if (wg_sub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
                 }
             }
             __syncthreads();

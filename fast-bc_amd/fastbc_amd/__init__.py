@@ -80,6 +80,17 @@ def lib():
                                       ctypes.c_int, _f64p]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        L.fbc_louvain.argtypes = [ctypes.c_uint64, i32p, i32p, _f64p, _u64p, ctypes.c_int, ctypes.c_double,
+                                  i32p]
+        L.fbc_select_pivots.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, i32p, _f64p, u8p, i32p, i32p]
+        L.fbc_kmeans.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, i32p, i32p, ctypes.c_double,
+                                 ctypes.c_int, i32p, i32p]
+        L.fbc_clustered_bc.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, _u64p, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _f64p, i32p, _f64p,
+                                       _u32p, _u32p, _u32p, _f64p]
+        L.fbc_exact_bc_arcs.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, ctypes.c_int, _f64p]
         _lib = L
     return _lib
 
@@ -268,6 +279,83 @@ class Clusters:
             self.graph.ctx.h, self.h, p, _u32(weights), _u32(pivot_cluster), len(p),
             np.ascontiguousarray(intra_bc, np.float64), prec, out))
         return out
+
+
+# ---- host engine (include/fbc_host.h) ---------------------------------------------
+
+def _arcs_i32(src, dst, w):
+    return (np.ascontiguousarray(src, np.int32), np.ascontiguousarray(dst, np.int32),
+            np.ascontiguousarray(w, np.float64))
+
+
+def _host_check(rc, what):
+    if rc < 0:
+        raise FbcError(rc, what)
+    return rc
+
+
+def louvain(src, dst, w, seeds, precision=0.01):
+    """Community of every vertex (LouvainGraphPartition::partitionGraph)."""
+    s, d, ww = _arcs_i32(src, dst, w)
+    n = int(s.max()) + 1 if len(s) else 0
+    comm = np.zeros(max(n, 1), np.int32)
+    k = _host_check(lib().fbc_louvain(len(s), s, d, ww, np.ascontiguousarray(seeds, np.uint64), len(seeds),
+                                      precision, comm), "fbc_louvain")
+    return comm[:n], k
+
+
+def select_pivots(vi_len, vi_cnt, bc_row, is_border):
+    """VertexInfoPivotSelector on one cluster's rows; returns (rows, weights, normalized len)."""
+    L = np.ascontiguousarray(vi_len, np.float64).copy()
+    nv, nb = L.shape
+    C = np.ascontiguousarray(vi_cnt, np.int32)
+    rows = np.zeros(max(nv, 1), np.int32)
+    wts = np.zeros(max(nv, 1), np.int32)
+    k = _host_check(lib().fbc_select_pivots(nv, nb, L.reshape(-1), C.reshape(-1),
+                                            np.ascontiguousarray(bc_row, np.float64),
+                                            np.ascontiguousarray(is_border, np.uint8), rows, wts),
+                    "fbc_select_pivots")
+    return rows[:k], wts[:k], L
+
+
+def kmeans(k, vi_len, vi_cnt, weights, min_variance=0.0, max_iteration=100):
+    L = np.ascontiguousarray(vi_len, np.float64)
+    P, nb = L.shape
+    c = np.zeros(k, np.int32)
+    cw = np.zeros(k, np.int32)
+    _host_check(lib().fbc_kmeans(k, P, nb, L.reshape(-1), np.ascontiguousarray(vi_cnt, np.int32).reshape(-1),
+                                 np.ascontiguousarray(weights, np.int32), min_variance, max_iteration, c, cw),
+                "fbc_kmeans")
+    return c, cw
+
+
+PHASES = ("louvain_s", "clusters_s", "pivot_selection_s", "pivot_phase_s", "total_s",
+          "cluster_te", "pivot_te", "pivot_kernel_ms")
+
+
+def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, device=0):
+    """End-to-end ClusteredBrandeBC::computeBC on the GPU; returns a dict."""
+    s, d, ww = _arcs_i32(src, dst, w)
+    n = int(s.max()) + 1
+    bc = np.zeros(n)
+    comm = np.zeros(n, np.int32)
+    intra = np.zeros(n)
+    piv = np.zeros(n, np.uint32)
+    pw = np.zeros(n, np.uint32)
+    pc = np.zeros(n, np.uint32)
+    ph = np.zeros(8)
+    k = _host_check(lib().fbc_clustered_bc(device, len(s), s, d, ww, np.ascontiguousarray(seeds, np.uint64),
+                                           len(seeds), precision, kfrac, prec, bc, comm, intra, piv, pw, pc, ph),
+                    "fbc_clustered_bc")
+    return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=pw[:k], pivot_cluster=pc[:k],
+                phases=dict(zip(PHASES, ph.tolist())))
+
+
+def exact_bc_arcs(src, dst, w, prec=PREC_F64, device=0):
+    s, d, ww = _arcs_i32(src, dst, w)
+    bc = np.zeros(int(s.max()) + 1)
+    _host_check(lib().fbc_exact_bc_arcs(device, len(s), s, d, ww, prec, bc), "fbc_exact_bc_arcs")
+    return bc
 
 
 # ---- generators ---------------------------------------------------------------

@@ -1,0 +1,61 @@
+/*
+ * fbc_host.h -- C ABI of the host side of the clustered betweenness path.
+ *
+ * Louvain partitioning, pivot selection and k-means stay on the host (the
+ * partition must be bit-identical to the reference's); the end-to-end entry
+ * point drives the GPU through fbc_gpu.h.  Same conventions as fbc_gpu.h:
+ * 0 or a negative FBC_E_* status, caller-owned buffers, blocking calls.
+ * Graphs are arc lists in file order with the loader's semantics
+ * (DirectedWeightedGraph.h:67-96,135-170): repeated arcs are summed, the
+ * vertex count is 1 + the largest source id.
+ */
+#ifndef FBC_HOST_C_H
+#define FBC_HOST_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* LouvainGraphPartition::partitionGraph (louvain/LouvainGraphPartition.h:84-130):
+ * comm_of[n] receives each vertex's community; returns the community count.
+ * Seeds are deduplicated and used in ascending order, one instance each. */
+int fbc_louvain(uint64_t m, const int32_t *src, const int32_t *dst, const double *w,
+                const uint64_t *seeds, int nseeds, double precision, int32_t *comm_of);
+
+/* VertexInfoPivotSelector::selectPivots (brandes/VertexInfoPivotSelector.h:29-127)
+ * on the nv x nb VertexInfo rows of one cluster (row r = r-th vertex of the
+ * ascending vertex list).  vi_len is normalized in place.  Fills piv_rows /
+ * weights (capacity nv); returns the pivot count. */
+int fbc_select_pivots(int nv, int nb, double *vi_len, const int32_t *vi_cnt,
+                      const double *bc_row, const uint8_t *is_border, int32_t *piv_rows,
+                      int32_t *weights);
+
+/* PlusPlusKMeans::computeCentroids (kmeans/PlusPlusKMeans.h:70-170) over P
+ * rows; fills centroids / cweights (capacity k), returns k. */
+int fbc_kmeans(int k, int P, int nb, const double *vi_len, const int32_t *vi_cnt,
+               const int32_t *weights, double min_variance, int max_iteration,
+               int32_t *centroids, int32_t *cweights);
+
+/* ClusteredBrandeBC::computeBC (brandes/ClusteredBrandesBC.h:63-169) as main.cpp
+ * composes it (main.cpp:223-260): kfrac <= 0 selects the exact pivot selector.
+ * bc[n] (pre-clamp), optional comm_of[n], intra_bc[n], pivots / weights /
+ * pivot_cluster (capacity n), phase_seconds[8] = {louvain, clusters, pivot
+ * selection, pivot phase, total, cluster traversed edges, pivot traversed
+ * edges, pivot-phase kernel ms}.  Returns the pivot count. */
+int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *dst,
+                     const double *w, const uint64_t *seeds, int nseeds, double precision,
+                     double kfrac, int prec, double *bc, int32_t *comm_of, double *intra_bc,
+                     uint32_t *pivots, uint32_t *weights, uint32_t *pivot_cluster,
+                     double *phase_seconds);
+
+/* ExactBrandesBC::computeBC (brandes/ExactBrandesBC.h:46-95) from arcs. */
+int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,
+                      const double *w, int prec, double *bc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

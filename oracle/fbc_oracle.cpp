@@ -244,20 +244,28 @@ void evaluate_cluster(const Cluster &c, std::vector<double> &bc,
 
 /* ---- VertexInfo arithmetic (brandes/VertexInfo.h) -------------------------- */
 
-/* VertexInfo<int,double>::squaredDistance of two points (VertexInfo.h:293-307),
+/* VertexInfo<V,W>::squaredDistance of two points (VertexInfo.h:293-307),
  * sequential order: per border the length term, then the count term.
- * pow(x, 2) is evaluated as x * x. */
-double sqdist_points(const double *la, const int *ca, const double *lb, const int *cb, int nb)
+ * pow(x, 2) is evaluated as x * x.  W = double is the instance the path uses;
+ * W = float is exported only so the reference's float known answer
+ * (test/brandes/VertexInfo.cpp:130) can check the order of operations. */
+template <typename W, typename WB>
+W sqdist_points_t(const W *la, const int *ca, const WB *lb, const int *cb, int nb)
 {
-    double sq = 0;
+    W sq = 0;
     for (int i = 0; i < nb; ++i) {
-        double dl = la[i] - lb[i];
+        W dl = la[i] - (W)lb[i];
         sq += dl * dl;
         int dc = (int)((uint32_t)ca[i] - (uint32_t)cb[i]);
-        double dcd = (double)dc;
+        W dcd = (W)dc;
         sq += dcd * dcd;
     }
     return sq;
+}
+
+double sqdist_points(const double *la, const int *ca, const double *lb, const int *cb, int nb)
+{
+    return sqdist_points_t<double, double>(la, ca, lb, cb, nb);
 }
 
 /* VertexInfo<double,double> (a mean) against a point. */
@@ -1073,6 +1081,16 @@ int orc_kmeans(int k, int P, int nb, const double *vi_len, const int *vi_cnt,
         out_weights[i] = w[i];
     }
     return (int)cent.size();
+}
+
+float orc_vi_sqdist_f32(int nb, const float *len_a, const int *cnt_a, const int *len_b, const int *cnt_b)
+{
+    return sqdist_points_t<float, int>(len_a, cnt_a, len_b, cnt_b, nb);
+}
+
+double orc_vi_sqdist(int nb, const double *len_a, const int *cnt_a, const double *len_b, const int *cnt_b)
+{
+    return sqdist_points(len_a, cnt_a, len_b, cnt_b, nb);
 }
 
 int orc_louvain(const orc_graph *g, int nseeds, const uint64_t *seeds, double precision,

@@ -80,6 +80,14 @@ int orc_kmeans(int k, int P, int nb, const double *vi_len, const int *vi_cnt,
                const int *weights, double min_variance, int max_iteration,
                int *centroids, int *out_weights);
 
+/* VertexInfo::squaredDistance (brandes/VertexInfo.h:293-307) of two rows:
+ * the FP64 instance the path uses, and an FP32 instance (lengths of the
+ * second row given as int, as VertexInfo<short,int> in the reference's test). */
+double orc_vi_sqdist(int nb, const double *len_a, const int *cnt_a, const double *len_b,
+                     const int *cnt_b);
+float orc_vi_sqdist_f32(int nb, const float *len_a, const int *cnt_a, const int *len_b,
+                        const int *cnt_b);
+
 /* LouvainGraphPartition::partitionGraph (louvain/LouvainGraphPartition.h:84-130).
  * seeds are deduplicated and sorted as a std::set; comm_of[n] receives each
  * vertex's community index.  Returns the community count. */
