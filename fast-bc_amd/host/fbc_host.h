@@ -33,6 +33,7 @@ struct Graph {
     std::vector<double> rw;
     std::vector<double> in_wdeg, out_wdeg;
     double total_weight = 0;      /* sum of all arc weights in input order */
+    uint64_t id = 0;              /* unique per built graph (device upload cache key) */
 };
 
 /* Edge-list text loader.  Throws std::invalid_argument with the reference's
@@ -42,6 +43,9 @@ Graph load_edge_list(std::istream &in);
 Graph load_edge_list_file(const std::string &path);
 /* Same semantics from arc arrays (file order). */
 Graph from_arcs(size_t m, const int *src, const int *dst, const double *w);
+/* The same with an explicit vertex count n (ids must be < n): the vertex set
+ * of an IGraph view, e.g. a SubGraph whose last vertex has no internal arc. */
+Graph from_arcs_n(int n, size_t m, const int *src, const int *dst, const double *w);
 void finish_graph(Graph &g);
 
 /* LouvainGraphPartition::partitionGraph (louvain/LouvainGraphPartition.h:84-130)
@@ -92,7 +96,7 @@ public:
 
 private:
     fbc_ctx *ctx_ = nullptr;
-    const Graph *cached_ = nullptr;
+    uint64_t cached_ = 0;
     fbc_graph *graph_ = nullptr;
     std::mutex mu_;
 };

@@ -13,6 +13,7 @@
 #include "fbc_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <climits>
 #include <cstdlib>
@@ -57,6 +58,8 @@ bool parse_int(const char *&p, const char *end, long long &out)
 
 void finish_graph(Graph &g)
 {
+    static std::atomic<uint64_t> next_id{1};
+    g.id = next_id++;
     const int n = g.n;
     /* backward CSR: counting sort of the forward arcs by target; rows end up
      * sorted by source because sources are visited in ascending order */
@@ -77,13 +80,19 @@ void finish_graph(Graph &g)
 
 Graph from_arcs(size_t m, const int *src, const int *dst, const double *w)
 {
+    return from_arcs_n(-1, m, src, dst, w);
+}
+
+Graph from_arcs_n(int n, size_t m, const int *src, const int *dst, const double *w)
+{
     Graph g;
     int maxsrc = -1;
     for (size_t i = 0; i < m; ++i) {
         if (src[i] < 0 || dst[i] < 0) throw std::invalid_argument("negative vertex id");
         maxsrc = std::max(maxsrc, src[i]);
     }
-    g.n = maxsrc + 1;
+    if (n >= 0 && maxsrc >= n) throw std::invalid_argument("arc source beyond the vertex count");
+    g.n = n >= 0 ? n : maxsrc + 1;
     for (size_t i = 0; i < m; ++i)
         if (dst[i] >= g.n)
             throw std::invalid_argument("arc target " + std::to_string(dst[i]) +

@@ -57,11 +57,11 @@ void Device::check(int rc) const
 
 fbc_graph *Device::graph(const Graph &g)
 {
-    if (cached_ == &g && graph_) return graph_;
+    if (cached_ == g.id && graph_) return graph_;
     if (graph_) {
         fbc_graph_free(graph_);
         graph_ = nullptr;
-        cached_ = nullptr;
+        cached_ = 0;
     }
     std::vector<uint32_t> col(g.col.size()), w(g.w.size());
     for (size_t e = 0; e < g.w.size(); ++e) {
@@ -72,7 +72,7 @@ fbc_graph *Device::graph(const Graph &g)
         col[e] = (uint32_t)g.col[e];
     }
     check(fbc_graph_upload(ctx_, (uint32_t)g.n, g.col.size(), g.rp.data(), col.data(), w.data(), &graph_));
-    cached_ = &g;
+    cached_ = g.id;
     return graph_;
 }
 

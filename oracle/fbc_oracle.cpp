@@ -1128,6 +1128,28 @@ int orc_clustered_bc(const orc_graph *g, int nseeds, const uint64_t *seeds, doub
     return np;
 }
 
+int orc_clustered_pivots(const orc_graph *g, int nseeds, const uint64_t *seeds, double precision,
+                         double kfrac, int threads, int *comm_of, double *intra_bc, int *pivots,
+                         int *weights, int *pivot_cluster)
+{
+    if (nseeds < 1 || threads < 1) return -1;
+    std::vector<double> out;
+    ClusteredRun run;
+    int np = clustered(*g, std::vector<uint64_t>(seeds, seeds + nseeds), precision, kfrac, threads, 0, out,
+                       run);
+    for (size_t c = 0; c < run.comms.size(); ++c)
+        for (int v : run.comms[c]) comm_of[v] = (int)c;
+    std::copy(run.intra.begin(), run.intra.end(), intra_bc);
+    int k = 0;
+    for (size_t c = 0; c < run.piv.size(); ++c)
+        for (size_t p = 0; p < run.piv[c].size(); ++p, ++k) {
+            pivots[k] = run.piv[c][p];
+            weights[k] = run.piv_w[c][p];
+            pivot_cluster[k] = (int)c;
+        }
+    return np;
+}
+
 int orc_clustered_bc_sampled(const orc_graph *g, int nseeds, const uint64_t *seeds,
                              double precision, double kfrac, int threads, int max_pivots,
                              double *bc, double *phase_seconds)

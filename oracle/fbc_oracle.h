@@ -105,6 +105,12 @@ int orc_clustered_bc(const orc_graph *g, int nseeds, const uint64_t *seeds,
                      int *comm_of, double *intra_bc, int *pivots, int *weights,
                      int *pivot_cluster, double *te);
 
+/* Phases 1-3 of the clustered run only (partition, intra-cluster BC, pivots);
+ * all outputs have capacity n.  Returns the pivot count. */
+int orc_clustered_pivots(const orc_graph *g, int nseeds, const uint64_t *seeds, double precision,
+                         double kfrac, int threads, int *comm_of, double *intra_bc, int *pivots,
+                         int *weights, int *pivot_cluster);
+
 /* Sampled pivot phase: only the first max_pivots pivots (in cluster order) are
  * run.  Used for bounded CPU timing. */
 int orc_clustered_bc_sampled(const orc_graph *g, int nseeds, const uint64_t *seeds,

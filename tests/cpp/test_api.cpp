@@ -1,0 +1,187 @@
+/*
+ * The reference-named C++ API (fast-bc_amd/host/include) used the way the
+ * reference's own tests and main.cpp use libfastbc.  Known answers are the
+ * reference's (the .cpp files of libfastbc/test).  'test_api cpu <dwg9> <cfg0> <louvain0>'
+ * runs the host-only checks; 'test_api gpu <dwg9> <cfg0>' the device ones.
+ */
+#include <DirectedWeightedGraph.h>
+#include <SubGraph.h>
+#include <brandes/ClusteredBrandesBC.h>
+#include <brandes/DijkstraClusterEvaluator.h>
+#include <brandes/DijkstraSSBrandesBC.h>
+#include <brandes/ExactBrandesBC.h>
+#include <brandes/KMeansPivotSelector.h>
+#include <brandes/VertexInfoPivotSelector.h>
+#include <kmeans/PlusPlusKMeans.h>
+#include <louvain/LouvainGraphPartition.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <stdexcept>
+
+using namespace fastbc;
+using namespace fastbc::brandes;
+
+static int failures = 0;
+#define CHECK(cond)                                                             \
+    do {                                                                        \
+        if (!(cond)) {                                                          \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);         \
+            ++failures;                                                         \
+        }                                                                       \
+    } while (0)
+
+static std::shared_ptr<DirectedWeightedGraph<int, double>> load(const char *p)
+{
+    std::ifstream in(p);
+    if (!in.is_open()) throw std::runtime_error(std::string("cannot open ") + p);
+    return std::make_shared<DirectedWeightedGraph<int, double>>(in);
+}
+
+static void cpu_checks(const char *dwg9, const char *cfg0, const char *louvain0)
+{
+    auto g = load(dwg9);                                       /* DirectedWeightedGraph.cpp:23-40 */
+    CHECK(g->vertices().size() == 9 && g->edges() == 16);
+    CHECK(g->forwardStar(4).size() == 3 && g->forwardStar(4).at(5) == 1 && g->forwardStar(4).at(6) == 5 &&
+          g->forwardStar(4).at(8) == 3);
+    CHECK(g->backwardStar(4).size() == 3 && g->backwardStar(4).at(0) == 7 && g->backwardStar(4).at(2) == 4);
+    CHECK(g->edge(7, 5) == 2 && g->edge(0, 1) == 4 && g->edge(1, 0) == 0);
+
+    SubGraph<int, double> sg({0, 1, 2, 3, 4}, g);              /* SubGraph.cpp:30-46 */
+    CHECK(sg.vertices().size() == 5 && sg.edges() == 7);
+    CHECK(sg.forwardStar(4).empty() && sg.backwardStar(4).size() == 3);
+    CHECK(sg.forwardStar(3).size() == 1 && sg.forwardStar(3).at(4) == 3);
+    CHECK(sg.borders().size() == 2 && sg.isBorder(3) && sg.isBorder(4) && sg.referenceGraph() == g);
+
+    VertexInfo<short, float> vi(4);                            /* VertexInfo.cpp:94-130, 318-381 */
+    const float L[4] = {2.3f, 5.2f, 1.1f, 4.7f};
+    const short C[4] = {2, 5, 1, 4};
+    for (int i = 0; i < 4; ++i) {
+        vi.setBorderSPLength(i, L[i]);
+        vi.setBorderSPCount(i, C[i]);
+    }
+    VertexInfo<short, int> viD(4);
+    const int LD[4] = {2, 5, 1, 4}, CD[4] = {3, 4, 1, 4};
+    for (int i = 0; i < 4; ++i) {
+        viD.setBorderSPLength(i, LD[i]);
+        viD.setBorderSPCount(i, (short)CD[i]);
+    }
+    CHECK(vi.squaredDistance(viD) == 2.62999964f);
+    bool threw = false;
+    try {
+        vi.getBorderSPLength(4);
+    } catch (const std::out_of_range &) {
+        threw = true;
+    }
+    CHECK(threw);
+    VertexInfo<short, float> viB(vi);
+    viB.setBorderSPLength(2, 2.1f);
+    VertexInfo<short, float> viC(vi);
+    viC += 1;
+    CHECK(vi == vi && !(vi == viB) && vi < viB && viB < viC && viC > vi && vi != viC);
+    VertexInfo<short, float> viN(vi);
+    viN.normalize();
+    CHECK(viN.getBorderSPLength(2) == 0.0f && viN.getBorderSPLength(0) == 2.3f - 1.1f);
+
+    std::vector<double> bc = {1, 2, 2, 1.5, 1, 3};             /* VertexInfoPivotSelector.cpp:16-74 */
+    std::vector<std::shared_ptr<VertexInfo<int, double>>> vis(6);
+    const double PL[5][3] = {{1, 2, 3}, {2, 1, 3}, {2, 3, 4}, {4, 3, 5}, {5, 1, 3}};
+    const int PC[5][3] = {{2, 1, 1}, {2, 2, 1}, {2, 1, 1}, {2, 2, 1}, {1, 1, 3}};
+    for (int v = 0; v < 6; ++v) vis[v] = std::make_shared<VertexInfo<int, double>>(3);
+    for (int v = 0; v < 5; ++v)
+        for (int j = 0; j < 3; ++j) {
+            vis[v]->setBorderSPLength(j, PL[v][j]);
+            vis[v]->setBorderSPCount(j, PC[v][j]);
+        }
+    VertexInfoPivotSelector<int, double> ps;
+    auto piv = ps.selectPivots(bc, vis, {0, 1, 2, 3, 4}, {});
+    CHECK(piv.first == std::vector<int>({0, 3, 4}) && piv.second == std::vector<int>({2, 2, 1}));
+
+    kmeans::PlusPlusKMeans<int, double> km;                     /* k = 1: member nearest the mean */
+    std::vector<std::shared_ptr<VertexInfo<int, double>>> kv(4);
+    const double KL[4] = {0, 10, 4, 5};
+    for (int v = 0; v < 4; ++v) {
+        kv[v] = std::make_shared<VertexInfo<int, double>>(1);
+        kv[v]->setBorderSPLength(0, KL[v]);
+    }
+    auto kc = km.computeCentroids(1, {0, 1, 2, 3}, {1, 1, 1, 1}, kv);
+    CHECK(kc.first == std::vector<int>({3}) && kc.second == std::vector<int>({4}));
+    threw = false;
+    try {
+        KMeansPivotSelector<int, double> bad(std::make_shared<VertexInfoPivotSelector<int, double>>(),
+                                             std::make_shared<kmeans::PlusPlusKMeans<int, double>>(), 1.5);
+    } catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw);
+
+    auto g0 = load(cfg0);                                       /* Louvain, reference partition */
+    louvain::LouvainGraphPartition<int, double> lp({4, 2, 3, 1}, 0.01);
+    auto comms = lp.partitionGraph(g0);
+    std::ifstream lin(louvain0);
+    std::vector<int> gold;
+    for (int x; lin >> x;) gold.push_back(x);
+    bool same = gold.size() == g0->vertices().size();
+    for (size_t c = 0; c < comms.size() && same; ++c)
+        for (int v : comms[c]) same = same && gold[v] == (int)c;
+    CHECK(same);
+}
+
+static void gpu_checks(const char *dwg9, const char *cfg0)
+{
+    auto full = load(dwg9);
+    auto sub = std::make_shared<SubGraph<int, double>>(std::vector<int>({0, 1, 2, 3, 4}), full);
+    ExactBrandesBC<int, double> ex;                             /* ExactBrandesBC.cpp:31-35 */
+    auto e = ex.computeBC(sub);
+    CHECK(e.size() == 5 && e[0] == 2.0 && e[1] == 2.0 && e[2] == 0.5 && e[3] == 1.0 && e[4] == 0.0);
+
+    std::vector<double> cbc(full->vertices().size(), 0.0);     /* DijkstraClusterEvaluator.cpp:38-74 */
+    std::vector<std::shared_ptr<VertexInfo<int, double>>> gvi(full->vertices().size());
+    DijkstraClusterEvaluator<int, double> ce;
+    ce.evaluateCluster(cbc, gvi, sub);
+    CHECK(cbc == std::vector<double>({2, 2, 0.5, 1, 0, 0, 0, 0, 0}));
+    CHECK(gvi[0]->getBorderSPCount(0) == 1 && gvi[0]->getBorderSPLength(0) == 5 && gvi[0]->getBorderSPCount(1) == 2 &&
+          gvi[0]->getBorderSPLength(1) == 7);
+    CHECK(gvi[4]->getBorderSPCount(0) == 0 && gvi[4]->getBorderSPLength(0) == 0 && gvi[4]->getBorderSPCount(1) == 1);
+
+    DijkstraSSBrandesBC<int, double> ssb;                       /* DijkstraSSBrandesBC.cpp:26-28 */
+    CHECK(ssb.singleSourceBrandes(0, full).size() == full->vertices().size());
+
+    /* main.cpp's 2-clustered composition vs the batched path on cfg0 */
+    auto g0 = load(cfg0);
+    std::set<std::mt19937::result_type> seeds = {1, 2, 3, 4};
+    ClusteredBrandeBC<int, double> plain(
+        std::make_shared<louvain::LouvainGraphPartition<int, double>>(seeds, 0.01),
+        std::make_shared<DijkstraClusterEvaluator<int, double>>(), std::make_shared<DijkstraSSBrandesBC<int, double>>(),
+        std::make_shared<KMeansPivotSelector<int, double>>(std::make_shared<VertexInfoPivotSelector<int, double>>(),
+                                                           std::make_shared<kmeans::PlusPlusKMeans<int, double>>(), 0.5));
+    auto a = plain.computeBC(g0);
+    GpuClusteredBrandesBC<int, double> batched(seeds, 0.01, 0.5);
+    auto b = batched.computeBC(g0);
+    double scale = 0, err = 0;
+    for (size_t v = 0; v < a.size(); ++v) {
+        scale = std::max(scale, std::fabs(a[v]));
+        err = std::max(err, std::fabs(a[v] - b[v]));
+    }
+    std::printf("composition vs batched on cfg0: max |diff| %.3e (max |bc| %.3e), %zu pivots\n", err, scale,
+                batched.lastRun().pivots.size());
+    CHECK(err <= 1e-9 * scale);
+}
+
+int main(int argc, char **argv)
+{
+    try {
+        if (argc == 5 && std::string(argv[1]) == "cpu") cpu_checks(argv[2], argv[3], argv[4]);
+        else if (argc == 4 && std::string(argv[1]) == "gpu") gpu_checks(argv[2], argv[3]);
+        else {
+            std::fprintf(stderr, "usage: test_api cpu <dwg9> <cfg0> <louvain0> | gpu <dwg9> <cfg0>\n");
+            return 2;
+        }
+    } catch (const std::exception &ex) {
+        std::printf("FAIL exception: %s\n", ex.what());
+        return 1;
+    }
+    std::printf("%s\n", failures ? "FAILED" : "OK");
+    return failures ? 1 : 0;
+}

@@ -54,6 +54,8 @@ def lib():
         L.orc_clustered_bc.argtypes = [P, ctypes.c_int, u64p, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, f64p, i32p, f64p, i32p, i32p, i32p,
                                        ctypes.POINTER(ctypes.c_double)]
+        L.orc_clustered_pivots.argtypes = [P, ctypes.c_int, u64p, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, i32p, f64p, i32p, i32p, i32p]
         L.orc_clustered_bc_sampled.argtypes = [P, ctypes.c_int, u64p, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, f64p]
         _lib = L
@@ -168,6 +170,20 @@ class Graph:
         assert k >= 0
         return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=w[:k],
                     pivot_cluster=pc[:k], te=te.value)
+
+
+def clustered_pivots(g, seeds, precision=0.01, kfrac=0.0, threads=1):
+    """Partition, intra-cluster BC and pivots of a clustered run (no pivot phase)."""
+    s = np.ascontiguousarray(seeds, np.uint64)
+    n = g.n
+    comm = np.zeros(n, np.int32)
+    intra = np.zeros(n)
+    piv = np.zeros(n, np.int32)
+    w = np.zeros(n, np.int32)
+    pc = np.zeros(n, np.int32)
+    k = lib().orc_clustered_pivots(g.h, len(s), s, precision, kfrac, threads, comm, intra, piv, w, pc)
+    assert k >= 0
+    return dict(comm=comm, intra=intra, pivots=piv[:k], weights=w[:k], pivot_cluster=pc[:k])
 
 
 def select_pivots(verts, vi_len, vi_cnt, bc_of_vert, borders):

@@ -1,0 +1,95 @@
+"""The drop-in surfaces: the reference-named C++ API (tests/cpp/test_api.cpp)
+and the fbc command line (grammar, exit codes, bc.txt format of main.cpp)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import fastbc_amd as F
+import oracle_lib as O
+from conftest import GOLDEN, ROOT
+
+API = os.path.join(ROOT, "tests", "cpp", "test_api")
+DWG9 = os.path.join(GOLDEN, "dwg9.txt")
+
+
+@pytest.fixture(scope="module")
+def cfg0_files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cfg0")
+    n, arcs = F.generate_config(0)
+    g = str(d / "cfg0.txt")
+    F.write_edge_list(n, arcs, g)
+    lv = str(d / "louvain0.txt")
+    np.savetxt(lv, np.load(os.path.join(GOLDEN, "louvain.npz"))["cfg0"], fmt="%d")
+    return g, lv
+
+
+def build_api():
+    if not os.path.exists(API):
+        subprocess.run(["make", "-s", "-C", os.path.dirname(API)], check=True)
+
+
+def test_cpp_api_host_side(cfg0_files):
+    build_api()
+    r = subprocess.run([API, "cpu", DWG9, *cfg0_files], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_api_device_side(cfg0_files):
+    build_api()
+    r = subprocess.run([API, "gpu", DWG9, cfg0_files[0]], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+def fbc(*args, cwd=None):
+    return subprocess.run([F.FBC_BIN, *args], capture_output=True, text=True, cwd=cwd, timeout=600)
+
+
+def test_cli_option_errors(tmp_path):
+    """main.cpp:93-104,123-181: -1 (255) for option errors, -2 (254) for an
+    existing output file."""
+    out = str(tmp_path / "bc.txt")
+    r = fbc("-e", "abc", DWG9)
+    assert r.returncode == 255 and r.stdout.startswith("invalid argument for -e: 'abc'")
+    assert fbc("-o", out).returncode == 255 and "Missing input file path" in fbc("-o", out).stdout
+    assert fbc("-s", "1,2", DWG9, "-o", out).returncode == 255          # -s without -e
+    r = fbc("-s", "1,1", "-e", "2", DWG9, "-o", out)
+    assert r.returncode == 255 and "Duplicate value" in r.stdout
+    r = fbc("-s", "1,2,3", "-e", "4", DWG9, "-o", out)
+    assert r.returncode == 255 and "different from louvain executors" in r.stdout
+    r = fbc("-k", "1.5", DWG9, "-o", out)
+    assert r.returncode == 255 and "Kfrac value must be in range 0-1." in r.stdout
+    open(out, "w").close()
+    r = fbc("-o", out, DWG9)
+    assert r.returncode == 254 and f'File "{out}" already existing' in r.stdout
+    r = fbc("--bogus", "-o", str(tmp_path / "x.txt"), str(tmp_path / "missing.txt"))
+    assert r.returncode == 255 and "error opening" in r.stdout     # unknown options are ignored
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["exact", "clustered", "2-clustered"])
+def test_cli_end_to_end(tmp_path, cfg0_files, mode):
+    """bc.txt: default ostream formatting, negatives written as 0
+    (main.cpp:282-294); values against the oracle."""
+    g = cfg0_files[0]
+    out, full = str(tmp_path / "bc.txt"), str(tmp_path / "bc_full.txt")
+    args = {"exact": ["--exact"], "clustered": ["-s", "1,2,3,4", "-e", "4"],
+            "2-clustered": ["-s", "1,2,3,4", "-e", "4", "-p", "0.01", "-k", "0.5"]}[mode]
+    r = fbc(*args, "-o", out, "--full-output", full, g)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "[  info   ] Loaded graph contains 4096 vertices and 16128 edges" in r.stdout
+    assert "Total computation time:" in r.stdout
+    vals = np.loadtxt(full)
+    og = O.Graph.from_file(g)
+    if mode == "exact":
+        ref = og.exact_bc(8)
+    else:
+        ref = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=0.5 if mode == "2-clustered" else 0.0, threads=1)["bc"]
+    scale = np.abs(ref).max()
+    assert np.abs(vals - ref).max() <= 1e-9 * scale
+    expect = "".join(("%g" % x if x >= 0 else "0") + "\n" for x in vals)
+    # %g and the default ostream format agree (6 significant digits)
+    assert open(out).read() == expect
