@@ -7,8 +7,10 @@
  */
 #include "fbc_gpu.h"
 #include "fbc_internal.h"
+#include "fbc_kmeans.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -735,6 +737,130 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
     dfree(d_taux);
     dfree(d_cptr);
     dfree(d_pw);
+    return rc;
+}
+
+int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
+                     const uint32_t *k, const double *features, const int32_t *weights, double min_variance,
+                     uint32_t max_iteration, uint32_t *centroids, int32_t *cweights)
+{
+    if (!ctx || (nsets && (!npoints || !nfeat || !k || !features || !weights || !centroids || !cweights)))
+        return fail(ctx, FBC_E_ARG, "null argument");
+    if (nsets == 0) return 0;
+    std::vector<uint64_t> xoff(nsets), poff(nsets), koff(nsets), moff(nsets);
+    uint64_t xs = 0, ps = 0, ks = 0, ms = 0;
+    for (uint32_t s = 0; s < nsets; ++s) {
+        if (npoints[s] == 0 || k[s] == 0 || k[s] > npoints[s]) return fail(ctx, FBC_E_ARG, "bad k-means set sizes");
+        xoff[s] = xs;
+        poff[s] = ps;
+        koff[s] = ks;
+        moff[s] = ms;
+        xs += (uint64_t)npoints[s] * nfeat[s];
+        ps += npoints[s];
+        ks += k[s];
+        ms += (uint64_t)k[s] * nfeat[s];
+    }
+    /* exactness precondition and the exact squared norms */
+    std::vector<double> xnorm(ps);
+    for (uint32_t s = 0; s < nsets; ++s) {
+        double mx = 0;
+        const double *X = features + xoff[s];
+        for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) {
+            const double x = X[i];
+            if (x != std::floor(x)) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features must be integers");
+            mx = std::max(mx, std::fabs(x));
+        }
+        if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0)
+            return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
+        for (uint32_t v = 0; v < npoints[s]; ++v) {
+            double q = 0;
+            for (uint32_t f = 0; f < nfeat[s]; ++f) q += X[(uint64_t)v * nfeat[s] + f] * X[(uint64_t)v * nfeat[s] + f];
+            xnorm[poff[s] + v] = q;
+        }
+    }
+    std::vector<uint32_t> tcl, tp0;
+    for (uint32_t s = 0; s < nsets; ++s)
+        for (uint32_t p0 = 0; p0 < npoints[s]; p0 += 64) {
+            tcl.push_back(s);
+            tp0.push_back(p0);
+        }
+    std::vector<uint32_t> P(npoints, npoints + nsets), D(nfeat, nfeat + nsets), K(k, k + nsets),
+        active(nsets, 1);
+    std::vector<double> X(features, features + xs);
+    std::vector<int32_t> W(weights, weights + ps);
+    fbcdev::KmParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.ncl = nsets;
+    p.min_variance = min_variance;
+    uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtcl = nullptr, *dtp0 = nullptr;
+    uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr;
+    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmean = nullptr;
+    int32_t *dW = nullptr, *dcw = nullptr;
+    uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
+             *dmem = nullptr, *dact = nullptr;
+    hipError_t e;
+    int rc = 0;
+    double ms_total = 0;
+    if ((e = upload(&dP, P, ctx->stream)) != hipSuccess || (e = upload(&dD, D, ctx->stream)) != hipSuccess ||
+        (e = upload(&dK, K, ctx->stream)) != hipSuccess || (e = upload(&dxo, xoff, ctx->stream)) != hipSuccess ||
+        (e = upload(&dpo, poff, ctx->stream)) != hipSuccess || (e = upload(&dko, koff, ctx->stream)) != hipSuccess ||
+        (e = upload(&dmo, moff, ctx->stream)) != hipSuccess || (e = upload(&dX, X, ctx->stream)) != hipSuccess ||
+        (e = upload(&dxn, xnorm, ctx->stream)) != hipSuccess || (e = upload(&dW, W, ctx->stream)) != hipSuccess ||
+        (e = upload(&dtcl, tcl, ctx->stream)) != hipSuccess || (e = upload(&dtp0, tp0, ctx->stream)) != hipSuccess ||
+        (e = upload(&dact, active, ctx->stream)) != hipSuccess || (e = dalloc(&dcd, ps)) != hipSuccess ||
+        (e = dalloc(&dmean, ms)) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
+        (e = dalloc(&dcent, ks)) != hipSuccess || (e = dalloc(&dnext, ks)) != hipSuccess ||
+        (e = dalloc(&dcnt, ks)) != hipSuccess || (e = dalloc(&dstart, ks)) != hipSuccess ||
+        (e = dalloc(&dasg, ps)) != hipSuccess || (e = dalloc(&dmem, ps)) != hipSuccess)
+        rc = hip_fail(ctx, e, "k-means setup");
+    if (!rc) {
+        p.P = dP;
+        p.D = dD;
+        p.K = dK;
+        p.xoff = dxo;
+        p.poff = dpo;
+        p.koff = dko;
+        p.moff = dmo;
+        p.X = dX;
+        p.xnorm = dxn;
+        p.weights = dW;
+        p.cdist = dcd;
+        p.cent = dcent;
+        p.next = dnext;
+        p.cnt = dcnt;
+        p.start = dstart;
+        p.assign = dasg;
+        p.members = dmem;
+        p.mean = dmean;
+        p.cweights = dcw;
+        p.active = dact;
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        e = fbcdev::km_launch_init(p, ctx->stream);
+        for (uint32_t it = 1; e == hipSuccess && it <= max_iteration + 1; ++it) {
+            e = fbcdev::km_launch_iteration(p, (uint32_t)tcl.size(), dtcl, dtp0, it, max_iteration, ctx->stream);
+            if (e != hipSuccess) break;
+            e = hipMemcpyAsync(active.data(), dact, nsets * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            bool any = false;
+            for (uint32_t a : active) any = any || a;
+            if (!any) break;
+        }
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        float ms = 0;
+        if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        ms_total = ms;
+        if (e == hipSuccess) e = hipMemcpy(centroids, dcent, ks * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(cweights, dcw, ks * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "k-means run");
+    }
+    for (auto *q : {dP, dD, dK, dtcl, dtp0, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
+    for (auto *q : {dxo, dpo, dko, dmo}) dfree(q);
+    for (auto *q : {dX, dxn, dcd, dmean}) dfree(q);
+    dfree(dW);
+    dfree(dcw);
+    ctx->last_ms = ms_total;
+    ctx->last_te = 0;
     return rc;
 }
 

@@ -1,0 +1,38 @@
+/* Shared between kmeans.hip and the C-ABI layer (not a public header). */
+#ifndef FBC_KMEANS_H
+#define FBC_KMEANS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbcdev {
+
+/* A batch of independent k-means problems (one per cluster); all arrays are
+ * concatenations indexed through the per-set offsets. */
+struct KmParams {
+    uint32_t ncl;
+    const uint32_t *P, *D, *K;          /* points, features, centroids per set */
+    const uint64_t *xoff;               /* row block of the set in X (P * D doubles) */
+    const uint64_t *poff;               /* per-point arrays */
+    const uint64_t *koff;               /* per-centroid arrays */
+    const uint64_t *moff;               /* mean block (K * D doubles) */
+    const double *X;                    /* features, row-major */
+    const double *xnorm;                /* squared norm of every row (exact) */
+    const int32_t *weights;             /* per point */
+    double *cdist;                      /* per point, initialisation */
+    uint32_t *cent, *next, *cnt, *start;/* per centroid */
+    uint32_t *assign, *members;         /* per point */
+    double *mean;
+    int32_t *cweights;                  /* per centroid, final */
+    uint32_t *active;                   /* per set: 1 while iterating */
+    double min_variance;
+};
+
+hipError_t km_launch_init(const KmParams &p, hipStream_t s);
+hipError_t km_launch_iteration(const KmParams &p, uint32_t ntiles, const uint32_t *tile_cl,
+                               const uint32_t *tile_p0, uint32_t iteration, uint32_t max_iteration,
+                               hipStream_t s);
+
+} // namespace fbcdev
+
+#endif

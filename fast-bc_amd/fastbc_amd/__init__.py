@@ -24,8 +24,11 @@ EXPORTS = (
     "fbc_graph_free", "fbc_sssp_sigma", "fbc_single_source_dep", "fbc_dep_batch",
     "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
-    "fbc_last_counters",
+    "fbc_last_counters", "fbc_kmeans_batch",
 )
+
+# Every symbol include/fbc_host.h declares.
+HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_exact_bc_arcs")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels")
 
@@ -80,6 +83,9 @@ def lib():
                                       ctypes.c_int, _f64p]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
+        L.fbc_kmeans_batch.argtypes = [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _f64p,
+                                       np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), ctypes.c_double,
+                                       ctypes.c_uint32, _u32p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
         L.fbc_louvain.argtypes = [ctypes.c_uint64, i32p, i32p, _f64p, _u64p, ctypes.c_int, ctypes.c_double,
@@ -139,6 +145,23 @@ class Context:
 
     def graph(self, n, row_ptr, col, w):
         return Graph(self, n, row_ptr, col, w)
+
+    def kmeans_batch(self, sets, min_variance=0.0, max_iteration=100):
+        """PlusPlusKMeans on the GPU for a list of (k, features [P x D], weights)."""
+        np_ = _u32([len(f) for _, f, _ in sets])
+        nf = _u32([f.shape[1] for _, f, _ in sets])
+        kk = _u32([k for k, _, _ in sets])
+        feat = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float64).reshape(-1) for _, f, _ in sets]))
+        w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for _, _, x in sets]), np.int32)
+        oc = np.zeros(int(kk.sum()), np.uint32)
+        ow = np.zeros(int(kk.sum()), np.int32)
+        self.check(lib().fbc_kmeans_batch(self.h, len(sets), np_, nf, kk, feat, w, min_variance,
+                                          max_iteration, oc, ow))
+        out, o = [], 0
+        for k in kk:
+            out.append((oc[o:o + k].astype(np.int64), ow[o:o + k].astype(np.int64)))
+            o += int(k)
+        return out
 
 
 class Graph:

@@ -102,7 +102,8 @@ private:
 };
 
 struct PhaseTimes {
-    double louvain = 0, cluster = 0, pivots = 0, pivot_phase = 0, total = 0;
+    double louvain = 0, cluster = 0, pivots = 0, pivot_phase = 0, total = 0, kmeans = 0;
+    size_t kmeans_gpu_sets = 0, kmeans_host_sets = 0;
     double cluster_te = 0, pivot_te = 0, cluster_kernel_ms = 0, pivot_kernel_ms = 0;
     size_t clusters = 0, bordered = 0, pivot_count = 0;
 };

@@ -16,8 +16,11 @@
  */
 #include "fbc_host.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <numeric>
+#include <set>
 #include <stdexcept>
 
 namespace fbchost {
@@ -27,6 +30,81 @@ namespace {
 double now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+/* KMeansPivotSelector (KMeansPivotSelector.h:79-132) for every cluster:
+ * k = max((int)(P * kfrac), 1) over the P exact pivots.  Sets whose distances
+ * are exact in FP64 run as one GPU batch (fbc_kmeans_batch); the others on the
+ * host.  Repeated centroids are dropped keeping the first occurrence and its
+ * weight; prow / pw are replaced by the k-means pivots. */
+void kmeans_stage(Device &dev, uint32_t ncl, const std::vector<uint32_t> &bptr, double kfrac,
+                  std::vector<std::vector<int>> &prow, std::vector<std::vector<int>> &pw,
+                  const std::vector<std::vector<double>> &plen, const std::vector<std::vector<int>> &pcnt,
+                  PhaseTimes &t)
+{
+    const double t0 = now();
+    std::vector<uint32_t> gsets, hsets;
+    for (uint32_t c = 0; c < ncl; ++c) {
+        if (prow[c].empty()) continue;
+        const uint32_t nb = bptr[c + 1] - bptr[c];
+        double mx = 0;
+        for (double x : plen[c]) mx = std::max(mx, std::fabs(x));
+        for (int x : pcnt[c]) mx = std::max(mx, std::fabs((double)x));
+        (2.0 * 2 * nb * mx * mx < 9007199254740992.0 ? gsets : hsets).push_back(c);
+    }
+    std::vector<std::vector<int>> cent(ncl), cw(ncl);
+    if (!gsets.empty()) {
+        std::vector<uint32_t> np, nf, kk;
+        std::vector<double> feat;
+        std::vector<int32_t> w;
+        for (uint32_t c : gsets) {
+            const uint32_t P = (uint32_t)prow[c].size(), nb = bptr[c + 1] - bptr[c];
+            np.push_back(P);
+            nf.push_back(2 * nb);
+            kk.push_back((uint32_t)std::max((int)(P * kfrac), 1));
+            for (uint32_t p = 0; p < P; ++p)
+                for (uint32_t j = 0; j < nb; ++j) {
+                    feat.push_back(plen[c][(size_t)p * nb + j]);
+                    feat.push_back((double)pcnt[c][(size_t)p * nb + j]);
+                }
+            w.insert(w.end(), pw[c].begin(), pw[c].end());
+        }
+        const size_t ktot = std::accumulate(kk.begin(), kk.end(), (size_t)0);
+        std::vector<uint32_t> oc(ktot);
+        std::vector<int32_t> ow(ktot);
+        dev.check(fbc_kmeans_batch(dev.ctx(), (uint32_t)gsets.size(), np.data(), nf.data(), kk.data(), feat.data(),
+                                   w.data(), 0.0, 100, oc.data(), ow.data()));
+        size_t o = 0;
+        for (size_t i = 0; i < gsets.size(); ++i) {
+            cent[gsets[i]].assign(oc.begin() + o, oc.begin() + o + kk[i]);
+            cw[gsets[i]].assign(ow.begin() + o, ow.begin() + o + kk[i]);
+            o += kk[i];
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < hsets.size(); ++i) {
+        const uint32_t c = hsets[i], nb = bptr[c + 1] - bptr[c];
+        const int P = (int)prow[c].size();
+        kmeans(std::max((int)(P * kfrac), 1), P, (int)nb, plen[c].data(), pcnt[c].data(), pw[c].data(), 0.0, 100,
+               cent[c], cw[c]);
+    }
+    for (uint32_t c = 0; c < ncl; ++c) {
+        if (prow[c].empty()) continue;
+        std::vector<int> rows, w;
+        std::set<int> seen;
+        for (size_t i = 0; i < cent[c].size(); ++i) {
+            const int r = prow[c][cent[c][i]];
+            if (seen.insert(r).second) {
+                rows.push_back(r);
+                w.push_back(cw[c][i]);
+            }
+        }
+        prow[c].swap(rows);
+        pw[c].swap(w);
+    }
+    t.kmeans = now() - t0;
+    t.kmeans_gpu_sets = gsets.size();
+    t.kmeans_host_sets = hsets.size();
 }
 
 } // namespace
@@ -139,9 +217,12 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     fbc_last_stats(dev.ctx(), &run.t.cluster_kernel_ms, &run.t.cluster_te);
     run.t.cluster = now() - t0;
 
-    /* pivots per cluster (the selectors only read their own cluster) */
+    /* exact pivots per cluster (the selectors only read their own cluster);
+     * the normalized rows of the exact pivots are kept for k-means */
     t0 = now();
-    std::vector<std::vector<int>> piv(ncl), pw(ncl);
+    std::vector<std::vector<int>> piv(ncl), pw(ncl), prow(ncl);
+    std::vector<std::vector<double>> plen(ncl);
+    std::vector<std::vector<int>> pcnt(ncl);
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t c = 0; c < ncl; ++c) {
         const int nv = (int)(cptr[c + 1] - cptr[c]);
@@ -162,19 +243,20 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
         }
         std::vector<double> bcr((size_t)nv);
         for (int r = 0; r < nv; ++r) bcr[r] = run.intra[verts[cptr[c] + r]];
-        std::vector<int> rows, w;
-        select_pivots(nv, nb, len.data(), cnt.data(), bcr.data(), isb.data(), rows, w);
+        select_pivots(nv, nb, len.data(), cnt.data(), bcr.data(), isb.data(), prow[c], pw[c]);
         if (opt.kfrac > 0) {
-            std::vector<int> kr, kw;
-            kmeans_pivots(rows, w, nb, len.data(), cnt.data(), opt.kfrac, kr, kw);
-            rows.swap(kr);
-            w.swap(kw);
-        }
-        for (size_t p = 0; p < rows.size(); ++p) {
-            piv[c].push_back((int)verts[cptr[c] + rows[p]]);
-            pw[c].push_back(w[p]);
+            plen[c].resize(prow[c].size() * nb);
+            pcnt[c].resize(prow[c].size() * nb);
+            for (size_t p = 0; p < prow[c].size(); ++p)
+                for (int j = 0; j < nb; ++j) {
+                    plen[c][p * nb + j] = len[(size_t)prow[c][p] * nb + j];
+                    pcnt[c][p * nb + j] = cnt[(size_t)prow[c][p] * nb + j];
+                }
         }
     }
+    if (opt.kfrac > 0) kmeans_stage(dev, ncl, bptr, opt.kfrac, prow, pw, plen, pcnt, run.t);
+    for (uint32_t c = 0; c < ncl; ++c)
+        for (size_t p = 0; p < prow[c].size(); ++p) piv[c].push_back((int)verts[cptr[c] + prow[c][p]]);
     for (uint32_t c = 0; c < ncl; ++c) {
         if (bptr[c + 1] > bptr[c]) ++run.t.bordered;
         for (size_t p = 0; p < piv[c].size(); ++p) {

@@ -132,6 +132,20 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
                     const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
                     const double *intra_bc, int prec, double *bc);
 
+/* PlusPlusKMeans::computeCentroids (kmeans/PlusPlusKMeans.h:70-229) for nsets
+ * independent point sets in one batch (one set per cluster's exact pivots, as
+ * KMeansPivotSelector calls it, brandes/KMeansPivotSelector.h:99-101).
+ * Set s has npoints[s] rows of nfeat[s] features (row-major; rows of all sets
+ * concatenated in 'features'), npoints[s] weights and k[s] centroids.
+ * Features are VertexInfo rows (length then count per border) and must be
+ * integers with 2 * nfeat * max|feature|^2 < 2^53, which makes every
+ * point-to-point distance exact (FBC_E_UNSUPPORTED otherwise).  Outputs, k[s]
+ * entries per set, concatenated: centroid point indices and their weights. */
+int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
+                     const uint32_t *k, const double *features, const int32_t *weights,
+                     double min_variance, uint32_t max_iteration, uint32_t *centroids,
+                     int32_t *cweights);
+
 /* Kernel time of the last launch of the entry points above, in milliseconds,
  * and the number of forward arcs it examined (traversed edges). */
 int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edges);

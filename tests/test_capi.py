@@ -8,8 +8,8 @@ import fastbc_amd as F
 from conftest import ROOT
 
 
-def declared_functions():
-    text = open(os.path.join(ROOT, "include", "fbc_gpu.h")).read()
+def declared_functions(header="fbc_gpu.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b(fbc_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
@@ -17,11 +17,12 @@ def declared_functions():
 
 def test_header_matches_binding_list():
     assert declared_functions() == sorted(F.EXPORTS)
+    assert declared_functions("fbc_host.h") == sorted(F.HOST_EXPORTS)
 
 
 def test_library_exports_all_symbols():
     lib = ctypes.CDLL(F.LIB_PATH)
-    for name in declared_functions():
+    for name in declared_functions() + declared_functions("fbc_host.h"):
         assert hasattr(lib, name), name
 
 

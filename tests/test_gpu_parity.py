@@ -138,6 +138,36 @@ def test_cfg0_pivot_phase(cfg0):
     assert (err <= 1e-9 * np.abs(ref) + 1e-13 * mag).all(), (err / np.maximum(mag, 1)).max()
 
 
+def interleave(L, C):
+    """VertexInfo rows as k-means features: length then count per border."""
+    L, C = np.asarray(L, float), np.asarray(C, float)
+    f = np.empty((L.shape[0], 2 * L.shape[1]))
+    f[:, 0::2], f[:, 1::2] = L, C
+    return f
+
+
+def test_kmeans_batch_matches_reference(gpu_ctx):
+    gold = np.load(os.path.join(GOLDEN, "kmeans_cases.npz"))
+    sets = [(int(gold[f"k{t}"][0]), interleave(gold[f"L{t}"], gold[f"C{t}"]), gold[f"W{t}"]) for t in range(24)]
+    for t, (cent, cw) in enumerate(gpu_ctx.kmeans_batch(sets)):
+        assert np.array_equal(cent, gold[f"cent{t}"]) and np.array_equal(cw, gold[f"cw{t}"]), t
+
+
+def test_kmeans_batch_matches_oracle(gpu_ctx):
+    rng = np.random.default_rng(77)
+    sets, expect = [], []
+    for t in range(12):
+        P, nb = int(rng.integers(2, 700)), int(rng.integers(1, 40))
+        k = int(rng.integers(1, P))
+        L = rng.integers(0, 3000 if t % 2 else 4, (P, nb)).astype(float)   # ties when values are small
+        C = rng.integers(0, 3, (P, nb)).astype(np.int32)
+        W = rng.integers(1, 9, P).astype(np.int32)
+        sets.append((k, interleave(L, C), W))
+        expect.append(O.kmeans(k, L, C, W))
+    for (cent, cw), (oc, ow) in zip(gpu_ctx.kmeans_batch(sets), expect):
+        assert np.array_equal(cent, oc) and np.array_equal(cw, ow)
+
+
 def test_sigma_overflow_detected(gpu_ctx):
     """Unit-weight bidirected 70x70 grid: sigma to the far corner is
     C(138, 69) > 2^64."""
