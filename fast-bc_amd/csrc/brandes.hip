@@ -28,49 +28,77 @@
  */
 #include "fbc_internal.h"
 
+#include <hip/hip_cooperative_groups.h>
+
 namespace fbcdev {
 
 constexpr int HEAVY_DEG = 96;   /* out-degree handled by the whole workgroup */
 constexpr int HEAVY_CAP = 64;
 constexpr int SUCC_REG = 8;     /* successors kept in registers by the ordered fold */
+constexpr int RB = 8;           /* arcs of one vertex whose loads and atomics are issued together */
 
 /* ---- memory helpers ------------------------------------------------------ */
 
-/* Per-slot state is touched by one workgroup only, so its atomics and the
- * loads of words other threads update atomically use workgroup scope: they
- * stay in the XCD's L2 instead of going to the device-coherent memory side.
- * Only the shared outputs (bc) and the task queue use device scope. */
-__device__ __forceinline__ uint32_t ld_a32(const uint32_t *p)
+/* Atomics and atomic loads take the scope of the team that owns the state:
+ * per-slot state of the batched kernel is touched by one workgroup only, so
+ * workgroup scope keeps it in the XCD's L2; the plain kernel spreads one
+ * source over the whole grid and needs device (agent) scope. */
+template <int SC> __device__ __forceinline__ uint32_t ald32(const uint32_t *p)
 {
-    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, SC);
 }
-__device__ __forceinline__ uint64_t ld_a64(const uint64_t *p)
+template <int SC> __device__ __forceinline__ uint64_t ald64(const uint64_t *p)
 {
-    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, SC);
 }
-__device__ __forceinline__ uint32_t wg_min(uint32_t *p, uint32_t v)
+template <int SC> __device__ __forceinline__ void ast32(uint32_t *p, uint32_t v)
 {
-    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, SC);
 }
-__device__ __forceinline__ uint32_t wg_max(uint32_t *p, uint32_t v)
+template <int SC> __device__ __forceinline__ void ast64(uint64_t *p, uint64_t v)
 {
-    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, SC);
 }
-__device__ __forceinline__ uint32_t wg_sub(uint32_t *p, uint32_t v)
+template <int SC> __device__ __forceinline__ uint32_t aadd32(uint32_t *p, uint32_t v)
 {
-    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, SC);
 }
-__device__ __forceinline__ uint64_t wg_add64(uint64_t *p, uint64_t v)
+template <int SC> __device__ __forceinline__ uint32_t amin32(uint32_t *p, uint32_t v)
 {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC> __device__ __forceinline__ uint32_t amax32(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC> __device__ __forceinline__ uint64_t aadd64(uint64_t *p, uint64_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC> __device__ __forceinline__ uint64_t asub64(uint64_t *p, uint64_t v)
+{
+    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC> __device__ __forceinline__ uint64_t amin64(uint64_t *p, uint64_t v)
+{
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC> __device__ __forceinline__ uint64_t amax64(uint64_t *p, uint64_t v)
+{
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, SC);
 }
 
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
+/* Per-source state.  key[v] packs the tentative distance (high 32 bits) with
+ * a tag (low 32 bits): during the distance phase the round in which v was
+ * last queued, afterwards its count of unprocessed DAG predecessors.  sd[2v]
+ * is sigma(v), sd[2v+1] delta(v) (a double, or a float pair, bit-cast), so a
+ * DAG successor's (sigma, delta) share one 16-byte line segment. */
 struct Slot {
-    uint32_t *dist, *stamp, *fstamp, *qa, *qb, *fa, *fb;
-    uint64_t *sigma;
-    uint64_t *delta;   /* double, or a float pair, bit-cast */
+    uint64_t *key;
+    uint32_t *fstamp, *qa, *qb, *fa, *fb;
+    uint64_t *sd;
 };
 
 __host__ __device__ inline uint64_t slot_layout(uint32_t nmax, uint8_t *base, Slot *s)
@@ -78,17 +106,36 @@ __host__ __device__ inline uint64_t slot_layout(uint32_t nmax, uint8_t *base, Sl
     const uint64_t w32 = align256((uint64_t)(nmax + 1) * 4);
     const uint64_t w64 = align256((uint64_t)(nmax + 1) * 8);
     if (s) {
-        s->dist = (uint32_t *)(base);
-        s->stamp = (uint32_t *)(base + w32);
-        s->fstamp = (uint32_t *)(base + 2 * w32);
-        s->qa = (uint32_t *)(base + 3 * w32);
-        s->qb = (uint32_t *)(base + 4 * w32);
-        s->fa = (uint32_t *)(base + 5 * w32);
-        s->fb = (uint32_t *)(base + 6 * w32);
-        s->sigma = (uint64_t *)(base + 7 * w32);
-        s->delta = (uint64_t *)(base + 7 * w32 + w64);
+        s->key = (uint64_t *)(base);
+        s->fstamp = (uint32_t *)(base + w64);
+        s->qa = (uint32_t *)(base + w64 + w32);
+        s->qb = (uint32_t *)(base + w64 + 2 * w32);
+        s->fa = (uint32_t *)(base + w64 + 3 * w32);
+        s->fb = (uint32_t *)(base + w64 + 4 * w32);
+        s->sd = (uint64_t *)(base + w64 + 5 * w32);
     }
-    return 7 * w32 + 2 * w64;
+    return 3 * w64 + 5 * w32;
+}
+
+/* distance half of key[v] */
+template <int SC> __device__ __forceinline__ uint32_t dist_of(const Slot S, uint32_t v)
+{
+    return ald32<SC>(reinterpret_cast<const uint32_t *>(S.key) + 2 * (uint64_t)v + 1);
+}
+/* the same once the key is no longer written (dependency phase) */
+__device__ __forceinline__ uint32_t dist_final(const Slot S, uint32_t v)
+{
+    return reinterpret_cast<const uint32_t *>(S.key)[2 * (uint64_t)v + 1];
+}
+template <int SC> __device__ __forceinline__ uint64_t sigma_of(const Slot S, uint32_t v)
+{
+    return ald64<SC>(&S.sd[2 * (uint64_t)v]);
+}
+/* (sigma, delta) of a DAG successor in one 16-byte load; valid once the
+ * sigma sweep and the successor's level are complete */
+__device__ __forceinline__ ulonglong2 sd_pair(const Slot S, uint32_t v)
+{
+    return *reinterpret_cast<const ulonglong2 *>(S.sd + 2 * (uint64_t)v);
 }
 
 uint64_t slot_bytes(uint32_t nmax) { return slot_layout(nmax, nullptr, nullptr); }
@@ -160,24 +207,59 @@ struct Counters {
     unsigned long long c[STAT_COUNT] = {};
 };
 
+/* ---- execution teams ------------------------------------------------------ */
+
+/* Counters of one source's traversal shared by all threads of its team: in
+ * LDS for a workgroup team, in device memory for the grid team. */
+struct Ctl {
+    uint32_t qcnt[3];     /* near-queue lengths by round mod 3 */
+    uint32_t fcnt[2];     /* far-list lengths by window parity */
+    uint32_t minv[2];     /* smallest far distance by window parity */
+    uint32_t tail;        /* level-list length */
+};
+
+/* Per-workgroup LDS. */
 struct Shared {
+    Ctl ctl;
     uint32_t task;
-    uint32_t qcnt[2];
-    uint32_t fcnt, fcnt2;
-    uint32_t minv;
-    uint32_t tail;
-    uint32_t nheavy;
+    uint32_t nheavy[2];   /* by round (level) parity */
     uint32_t heavy[HEAVY_CAP];
     unsigned long long te;
 };
 
-/* Relax arc e of u (distance du).  Pushes into the next near queue or the far
- * list. */
-__device__ __forceinline__ void relax_arc(const View &V, const Slot &S, Shared &sh,
-                                          uint32_t du, uint32_t e, uint32_t thr,
-                                          uint32_t nxt, uint32_t win, uint32_t *qnext,
-                                          uint32_t *far, uint32_t *err, Counters &ct)
+/* One workgroup per source (batched kernel). */
+struct TeamWG {
+    static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
+    Ctl *ctl;
+    __device__ uint32_t rank() const { return threadIdx.x; }
+    __device__ uint32_t size() const { return WG; }
+    __device__ bool leader() const { return threadIdx.x == 0; }
+    __device__ void sync() const { __syncthreads(); }
+    __device__ bool aborted() const { return false; }
+};
+
+/* The whole co-resident grid per source (plain kernel, launched with
+ * hipLaunchCooperativeKernel, which guarantees that every workgroup is
+ * resident, so the grid barrier always completes). */
+struct TeamGrid {
+    static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
+    Ctl *ctl;
+    __device__ uint32_t rank() const { return blockIdx.x * WG + threadIdx.x; }
+    __device__ uint32_t size() const { return gridDim.x * WG; }
+    __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
+    __device__ void sync() const { cooperative_groups::this_grid().sync(); }
+    __device__ bool aborted() const { return false; }
+};
+
+/* Relax arc e of u (distance du); used for the arcs of heavy vertices, one
+ * per thread. */
+template <class T>
+__device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t du, uint32_t e,
+                                          uint32_t thr, uint32_t nxt, uint32_t win, uint32_t *qnext,
+                                          uint32_t *qn_cnt, uint32_t *far, uint32_t *far_cnt,
+                                          uint32_t *err, Counters &ct)
 {
+    constexpr int SC = T::SC;
     ++ct.c[STAT_RELAX];
     const uint2 a = V.arcs[e];
     const uint64_t nd64 = (uint64_t)du + a.y;
@@ -186,43 +268,102 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot &S, Shared &
         return;
     }
     const uint32_t nd = (uint32_t)nd64;
-    if (nd >= ld_a32(&S.dist[a.x])) return;
-    const uint32_t old = wg_min(&S.dist[a.x], nd);
-    if (nd >= old) return;
+    /* near candidates carry the round tag, far ones tag 0: key order is the
+     * distance order, and an equal distance never replaces an older tag */
+    const uint64_t nk = ((uint64_t)nd << 32) | (nd < thr ? nxt : 0u);
+    if (nk >= ald64<SC>(&S.key[a.x])) return;
+    const uint64_t old = amin64<SC>(&S.key[a.x], nk);
+    if ((old >> 32) <= nd) return;
     ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
-        if (wg_max(&S.stamp[a.x], nxt) < nxt) qnext[atomicAdd(&sh.qcnt[nxt & 1], 1u)] = a.x;
-    } else {
-        if (wg_max(&S.fstamp[a.x], win) < win) {
-            far[atomicAdd(&sh.fcnt, 1u)] = a.x;
-            ++ct.c[STAT_FARPUSH];
+        /* queued already in this round (with a larger distance): not again */
+        if ((uint32_t)old != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = a.x;
+    } else if (amax32<SC>(&S.fstamp[a.x], win) < win) {
+        far[aadd32<SC>(far_cnt, 1u)] = a.x;
+        ++ct.c[STAT_FARPUSH];
+    }
+}
+
+/* Relax the arcs [e0, e1) of u (distance du), RB at a time: the arc loads,
+ * the key loads and the atomicMins of a block are independent, so they are
+ * issued back to back and their latencies overlap. */
+template <class T>
+__device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32_t du, uint32_t e0,
+                                             uint32_t e1, uint32_t thr, uint32_t nxt, uint32_t win,
+                                             uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far,
+                                             uint32_t *far_cnt, uint32_t *err, Counters &ct)
+{
+    constexpr int SC = T::SC;
+    ct.c[STAT_RELAX] += e1 - e0;
+    for (uint32_t eb = e0; eb < e1; eb += RB) {
+        uint32_t to[RB];
+        uint64_t nk[RB], cur[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            to[j] = 0;
+            nk[j] = ~uint64_t(0);
+            if (eb + j < e1) {
+                const uint2 a = V.arcs[eb + j];
+                const uint64_t nd64 = (uint64_t)du + a.y;
+                to[j] = a.x;
+                if (nd64 >= DINF) {
+                    atomicOr(err, ERR_DIST);
+                } else {
+                    const uint32_t nd = (uint32_t)nd64;
+                    nk[j] = ((uint64_t)nd << 32) | (nd < thr ? nxt : 0u);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RB; ++j) cur[j] = nk[j] != ~uint64_t(0) ? ald64<SC>(&S.key[to[j]]) : 0;
+#pragma unroll
+        for (int j = 0; j < RB; ++j) cur[j] = nk[j] < cur[j] ? amin64<SC>(&S.key[to[j]], nk[j]) : 0;
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            /* cur[j] is the key before our atomicMin (0 when none was issued) */
+            const uint32_t nd = (uint32_t)(nk[j] >> 32);
+            if ((cur[j] >> 32) <= nd) continue;
+            ++ct.c[STAT_IMPROVE];
+            if (nd < thr) {
+                if ((uint32_t)cur[j] != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = to[j];
+            } else if (amax32<SC>(&S.fstamp[to[j]], win) < win) {
+                far[aadd32<SC>(far_cnt, 1u)] = to[j];
+                ++ct.c[STAT_FARPUSH];
+            }
         }
     }
 }
 
-/* Phase 1: near-far delta-stepping.  On return dist[] is final. */
-__device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32_t src,
+/* Phase 1: near-far delta-stepping.  On return the distances are final.
+ *
+ * Round r expands the near queue counted by qcnt[r % 3] and fills the one
+ * counted by qcnt[(r + 1) % 3]; its leader clears qcnt[(r + 2) % 3], last
+ * read two rounds earlier, so one team barrier per round suffices.  Far-list
+ * counters and window minima alternate by window parity in the same way. */
+template <class T>
+__device__ __forceinline__ void phase_distances(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
                                 uint32_t delta, uint32_t *err, Counters &ct)
 {
-    const int tid = threadIdx.x;
-    for (uint32_t v = tid; v < V.n; v += WG) {
-        S.dist[v] = DINF;
-        S.stamp[v] = 0;
+    constexpr int SC = T::SC;
+    const uint32_t r0 = t.rank(), R = t.size();
+    Ctl &c = *t.ctl;
+    for (uint32_t v = r0; v < V.n; v += R) {
+        S.key[v] = v == src ? 1u : (uint64_t)DINF << 32;   /* src: distance 0, queued for round 1 */
         S.fstamp[v] = 0;
-        S.sigma[v] = 0;
+        S.sd[2 * (uint64_t)v] = 0;
     }
-    if (tid == 0) {
-        sh.qcnt[0] = sh.qcnt[1] = 0;
-        sh.fcnt = sh.fcnt2 = 0;
-        sh.nheavy = 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        S.dist[src] = 0;
-        S.stamp[src] = 1;
+    if (t.leader()) {
+        ast32<SC>(&c.qcnt[0], 0u);
+        ast32<SC>(&c.qcnt[1], 1u);
+        ast32<SC>(&c.qcnt[2], 0u);
+        ast32<SC>(&c.fcnt[0], 0u);
+        ast32<SC>(&c.fcnt[1], 0u);
+        ast32<SC>(&c.minv[0], DINF);
+        ast32<SC>(&c.minv[1], DINF);
         S.qa[0] = src;
     }
-    __syncthreads();
+    if (threadIdx.x == 0) sh.nheavy[0] = sh.nheavy[1] = 0;
+    t.sync();
 
     uint32_t *qa = S.qa, *qb = S.qb, *fa = S.fa, *fb = S.fb;
     uint32_t near = 1, rnd = 1, win = 1;
@@ -230,172 +371,194 @@ __device__ void phase_distances(const View &V, const Slot &S, Shared &sh, uint32
     for (;;) {
         while (near) {
             const uint32_t nxt = rnd + 1;
-            if (tid == 0) ++ct.c[STAT_ROUNDS];
-            for (uint32_t i = tid; i < near; i += WG) {
+            uint32_t *qn_cnt = &c.qcnt[nxt % 3];
+            uint32_t *far_cnt = &c.fcnt[win & 1];
+            uint32_t *nh_cnt = &sh.nheavy[rnd & 1];
+            if (t.leader()) {
+                ++ct.c[STAT_ROUNDS];
+                ast32<SC>(&c.qcnt[(rnd + 2) % 3], 0u);
+            }
+            if (threadIdx.x == 0) sh.nheavy[(rnd + 1) & 1] = 0;
+            for (uint32_t i = r0; i < near; i += R) {
                 const uint32_t u = qa[i];
                 ++ct.c[STAT_VISITS];
-                const uint32_t du = ld_a32(&S.dist[u]);
+                const uint32_t du = dist_of<SC>(S, u);
                 const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
                 if (e1 - e0 > (uint32_t)HEAVY_DEG) {
-                    uint32_t h = atomicAdd(&sh.nheavy, 1u);
+                    const uint32_t h = atomicAdd(nh_cnt, 1u);
                     if (h < (uint32_t)HEAVY_CAP) {
                         sh.heavy[h] = u;
                         continue;
                     }
                 }
-                for (uint32_t e = e0; e < e1; ++e)
-                    relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err, ct);
+                relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
             }
             __syncthreads();
-            const uint32_t nh = sh.nheavy < (uint32_t)HEAVY_CAP ? sh.nheavy : (uint32_t)HEAVY_CAP;
-            if (nh) {
-                for (uint32_t h = 0; h < nh; ++h) {
-                    const uint32_t u = sh.heavy[h];
-                    const uint32_t du = ld_a32(&S.dist[u]);
-                    const uint32_t e1 = V.rp[u + 1];
-                    for (uint32_t e = V.rp[u] + tid; e < e1; e += WG)
-                        relax_arc(V, S, sh, du, e, thr, nxt, win, qb, fa, err, ct);
-                }
-                __syncthreads();
+            const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
+            for (uint32_t h = 0; h < nh; ++h) {   /* heavy vertices: the workgroup splits the arcs */
+                const uint32_t u = sh.heavy[h];
+                const uint32_t du = dist_of<SC>(S, u);
+                const uint32_t e1 = V.rp[u + 1];
+                for (uint32_t e = V.rp[u] + threadIdx.x; e < e1; e += WG)
+                    relax_arc<T>(V, S, du, e, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
             }
-            near = sh.qcnt[nxt & 1];
-            uint32_t *t = qa; qa = qb; qb = t;
+            t.sync();
+            near = ald32<SC>(qn_cnt);
+            uint32_t *tq = qa; qa = qb; qb = tq;
             rnd = nxt;
-            __syncthreads();
-            if (tid == 0) {
-                sh.qcnt[(rnd + 1) & 1] = 0;
-                sh.nheavy = 0;
-            }
-            __syncthreads();
+            if (t.aborted()) return;
         }
-        /* window [.., thr) is settled; move on to the nearest far vertex */
-        const uint32_t nfar = sh.fcnt;
+        /* window [.., thr) is settled; move on to the nearest far vertex.  The
+         * near queue of round rnd is empty and its counter is 0. */
+        const uint32_t nfar = ald32<SC>(&c.fcnt[win & 1]);
         if (nfar == 0) break;
-        if (tid == 0) ++ct.c[STAT_WINDOWS];
-        if (tid == 0) sh.minv = DINF;
-        __syncthreads();
+        if (t.leader()) {
+            ++ct.c[STAT_WINDOWS];
+            ast32<SC>(&c.fcnt[(win + 1) & 1], 0u);   /* last read in the previous window */
+            ast32<SC>(&c.minv[(win + 1) & 1], DINF);
+        }
         uint32_t mn = DINF;
-        for (uint32_t i = tid; i < nfar; i += WG) {
-            const uint32_t d = ld_a32(&S.dist[fa[i]]);
+        for (uint32_t i = r0; i < nfar; i += R) {
+            const uint32_t d = dist_of<SC>(S, fa[i]);
             if (d >= thr && d < mn) mn = d;
         }
-        if (mn != DINF) atomicMin(&sh.minv, mn);
-        __syncthreads();
-        const uint32_t m = sh.minv;
+        if (mn != DINF) amin32<SC>(&c.minv[win & 1], mn);
+        t.sync();
+        const uint32_t m = ald32<SC>(&c.minv[win & 1]);
         if (m == DINF) break;
         const uint64_t nthr64 = (uint64_t)m + delta;
         const uint32_t nthr = nthr64 >= DINF ? DINF : (uint32_t)nthr64;
         const uint32_t nwin = win + 1;
-        for (uint32_t i = tid; i < nfar; i += WG) {
+        for (uint32_t i = r0; i < nfar; i += R) {
             const uint32_t w = fa[i];
-            const uint32_t d = ld_a32(&S.dist[w]);
+            const uint32_t d = dist_of<SC>(S, w);
             if (d < thr) continue;            /* settled in an earlier window */
             if (d < nthr) {
-                if (wg_max(&S.stamp[w], rnd) < rnd) qa[atomicAdd(&sh.qcnt[rnd & 1], 1u)] = w;
-            } else {
-                if (wg_max(&S.fstamp[w], nwin) < nwin) fb[atomicAdd(&sh.fcnt2, 1u)] = w;
+                /* far entries carry tag 0; tag the round they are queued for
+                 * (no relaxation runs during the split, so d is stable) */
+                if ((uint32_t)amax64<SC>(&S.key[w], ((uint64_t)d << 32) | rnd) < rnd)
+                    qa[aadd32<SC>(&c.qcnt[rnd % 3], 1u)] = w;
+            } else if (amax32<SC>(&S.fstamp[w], nwin) < nwin) {
+                fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
             }
         }
-        __syncthreads();
-        near = sh.qcnt[rnd & 1];
-        const uint32_t kept = sh.fcnt2;
-        uint32_t *t = fa; fa = fb; fb = t;
+        t.sync();
+        near = ald32<SC>(&c.qcnt[rnd % 3]);
+        uint32_t *tf = fa; fa = fb; fb = tf;
         thr = nthr;
         win = nwin;
-        __syncthreads();
-        if (tid == 0) {
-            sh.fcnt = kept;
-            sh.fcnt2 = 0;
+        if (t.aborted()) return;
+    }
+}
+
+/* Push sigma(u) along the DAG arcs among [e0, e1), RB at a time with the
+ * loads and atomics of a block in flight together; a successor whose last
+ * predecessor this was joins the level list. */
+template <class T>
+__device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t du, uint64_t su,
+                                           uint32_t e0, uint32_t e1, uint32_t step, uint32_t *tail,
+                                           uint32_t *err)
+{
+    constexpr int SC = T::SC;
+    for (uint32_t eb = e0; eb < e1; eb += RB * step) {
+        uint2 a[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) a[j] = eb + j * step < e1 ? V.arcs[eb + j * step] : make_uint2(0u, 0u);
+        bool tight[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+            tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dist_of<SC>(S, a[j].x);
+        uint64_t old[RB], cnt[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            old[j] = tight[j] ? aadd64<SC>(&S.sd[2 * (uint64_t)a[j].x], su) : 0;
+            cnt[j] = tight[j] ? asub64<SC>(&S.key[a[j].x], 1) : 0;
         }
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            if (!tight[j]) continue;
+            if (old[j] + su < old[j]) atomicOr(err, ERR_SIGMA);
+            if ((uint32_t)cnt[j] == 1u) S.qa[aadd32<SC>(tail, 1u)] = a[j].x;
+        }
     }
 }
 
 /* Phase 2: DAG in-degrees, then the Kahn sweep with exact sigma.  Returns the
- * number of levels; the level list is S.qa, the level starts S.fa. */
-__device__ uint32_t phase_sigma(const View &V, const Slot &S, Shared &sh, uint32_t src,
+ * number of levels; the level list is S.qa, the level starts S.fa.  The
+ * in-degree goes into the low half of key[v] (the round tags are done with);
+ * the high half keeps the distance. */
+template <class T>
+__device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
                                 uint32_t *err, Counters &ct)
 {
-    const int tid = threadIdx.x;
-    uint32_t *cnt = S.stamp;          /* stamps are no longer needed */
+    constexpr int SC = T::SC;
+    const uint32_t r0 = t.rank(), R = t.size();
+    Ctl &c = *t.ctl;
     unsigned long long te = 0;
-    for (uint32_t v = tid; v < V.n; v += WG) {
-        const uint32_t dv = ld_a32(&S.dist[v]);
-        uint32_t c = 0;
+    for (uint32_t v = r0; v < V.n; v += R) {
+        const uint32_t dv = dist_of<SC>(S, v);
+        uint32_t cin = 0;
         if (dv != DINF) {
             te += V.rp[v + 1] - V.rp[v];
             if (v != src) {
                 const uint32_t r1 = V.rrp[v + 1];
-                for (uint32_t r = V.rrp[v]; r < r1; ++r) {
-                    const uint2 a = V.rarcs[r];
-                    const uint32_t du = ld_a32(&S.dist[a.x]);
-                    if (du != DINF && (uint64_t)du + a.y == dv) ++c;
+                for (uint32_t rb = V.rrp[v]; rb < r1; rb += RB) {
+                    uint2 a[RB];
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) a[j] = rb + j < r1 ? V.rarcs[rb + j] : make_uint2(0u, 0u);
+                    uint32_t du[RB];
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) du[j] = rb + j < r1 ? dist_of<SC>(S, a[j].x) : DINF;
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) cin += du[j] != DINF && (uint64_t)du[j] + a[j].y == dv;
                 }
             }
         }
-        cnt[v] = c;
+        ast64<SC>(&S.key[v], ((uint64_t)dv << 32) | cin);
     }
     if (te) atomicAdd(&sh.te, te);
-    if (tid == 0) {
-        S.sigma[src] = 1;
+    if (t.leader()) {
+        S.sd[2 * (uint64_t)src] = 1;
         S.qa[0] = src;
         S.fa[0] = 0;
         S.fa[1] = 1;
-        sh.tail = 1;
-        sh.nheavy = 0;
+        ast32<SC>(&c.tail, 1u);
     }
-    __syncthreads();
+    if (threadIdx.x == 0) sh.nheavy[0] = sh.nheavy[1] = 0;
+    t.sync();
     uint32_t lb = 0, le = 1, nlev = 1;
-    while (lb < le) {
-        for (uint32_t i = lb + tid; i < le; i += WG) {
+    for (uint32_t lvl = 0; lb < le; ++lvl) {
+        uint32_t *nh_cnt = &sh.nheavy[lvl & 1];
+        if (threadIdx.x == 0) sh.nheavy[(lvl + 1) & 1] = 0;
+        for (uint32_t i = lb + r0; i < le; i += R) {
             const uint32_t u = S.qa[i];
-            const uint32_t du = ld_a32(&S.dist[u]);
-            const uint64_t su = ld_a64(&S.sigma[u]);
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
             if (e1 - e0 > (uint32_t)HEAVY_DEG) {
-                uint32_t h = atomicAdd(&sh.nheavy, 1u);
+                const uint32_t h = atomicAdd(nh_cnt, 1u);
                 if (h < (uint32_t)HEAVY_CAP) {
                     sh.heavy[h] = u;
                     continue;
                 }
             }
-            for (uint32_t e = e0; e < e1; ++e) {
-                const uint2 a = V.arcs[e];
-                if ((uint64_t)du + a.y != ld_a32(&S.dist[a.x])) continue;
-                const uint64_t old = wg_add64(&S.sigma[a.x], su);
-                if (old + su < old) atomicOr(err, ERR_SIGMA);
-                if (wg_sub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
-            }
+            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, &c.tail, err);
         }
         __syncthreads();
-        const uint32_t nh = sh.nheavy < (uint32_t)HEAVY_CAP ? sh.nheavy : (uint32_t)HEAVY_CAP;
-        if (nh) {
-            for (uint32_t h = 0; h < nh; ++h) {
-                const uint32_t u = sh.heavy[h];
-                const uint32_t du = ld_a32(&S.dist[u]);
-                const uint64_t su = ld_a64(&S.sigma[u]);
-                const uint32_t e1 = V.rp[u + 1];
-                for (uint32_t e = V.rp[u] + tid; e < e1; e += WG) {
-                    const uint2 a = V.arcs[e];
-                    if ((uint64_t)du + a.y != ld_a32(&S.dist[a.x])) continue;
-                    const uint64_t old = wg_add64(&S.sigma[a.x], su);
-                    if (old + su < old) atomicOr(err, ERR_SIGMA);
-                    if (wg_sub(&cnt[a.x], 1u) == 1u) S.qa[atomicAdd(&sh.tail, 1u)] = a.x;
-                }
-            }
-            __syncthreads();
+        const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
+        for (uint32_t h = 0; h < nh; ++h) {
+            const uint32_t u = sh.heavy[h];
+            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + threadIdx.x,
+                          V.rp[u + 1], WG, &c.tail, err);
         }
+        t.sync();
         lb = le;
-        le = sh.tail;
+        le = ald32<SC>(&c.tail);
         if (le > lb) {
             ++nlev;
-            if (tid == 0) S.fa[nlev] = le;
+            if (t.leader()) S.fa[nlev] = le;
         }
-        __syncthreads();
-        if (tid == 0) sh.nheavy = 0;
-        __syncthreads();
+        if (t.aborted()) return 0;
     }
-    if (tid == 0) ct.c[STAT_LEVELS] += nlev;
+    if (t.leader()) ct.c[STAT_LEVELS] += nlev;
     return nlev;
 }
 
@@ -438,53 +601,64 @@ struct Acc<1> {
 };
 
 /* Dependency of v: fold over DAG successors.  'ordered' sums them in
- * descending (d(w), w). */
+ * descending (d(w), w).  Distances, sigma and the successors' deltas are
+ * final here, so plain loads see them after the level barrier. */
 template <int PREC, bool ORDERED>
-__device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot &S, uint32_t v)
+__device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v)
 {
     Acc<PREC> acc;
-    const uint32_t dv = ld_a32(&S.dist[v]);
-    const uint64_t sv_int = ld_a64(&S.sigma[v]);
+    const uint32_t dv = dist_final(S, v);
+    const uint64_t sv_int = S.sd[2 * (uint64_t)v];
     const double sv = (double)sv_int;
     const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
     if (!ORDERED) {
-        for (uint32_t e = e0; e < e1; ++e) {
-            const uint2 a = V.arcs[e];
-            if ((uint64_t)dv + a.y != ld_a32(&S.dist[a.x])) continue;
-            acc.add(sv_int, sv, ld_a64(&S.sigma[a.x]), S.delta[a.x]);
+        /* ascending arc order, RB successors' loads in flight together */
+        for (uint32_t eb = e0; eb < e1; eb += RB) {
+            uint2 a[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j) a[j] = eb + j < e1 ? V.arcs[eb + j] : make_uint2(0u, 0u);
+            bool tight[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j) tight[j] = eb + j < e1 && (uint64_t)dv + a[j].y == dist_final(S, a[j].x);
+            ulonglong2 sw[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j) sw[j] = tight[j] ? sd_pair(S, a[j].x) : make_ulonglong2(0, 0);
+#pragma unroll
+            for (int j = 0; j < RB; ++j)
+                if (tight[j]) acc.add(sv_int, sv, sw[j].x, sw[j].y);
         }
         return acc;
     }
     if (e1 - e0 <= (uint32_t)SUCC_REG) {
+        uint2 a[SUCC_REG];
+#pragma unroll
+        for (int j = 0; j < SUCC_REG; ++j) a[j] = e0 + j < e1 ? V.arcs[e0 + j] : make_uint2(0u, 0u);
         uint64_t key[SUCC_REG];
-        uint32_t who[SUCC_REG];
 #pragma unroll
         for (int j = 0; j < SUCC_REG; ++j) {
             key[j] = 0;
-            who[j] = 0;
             if (e0 + j < e1) {
-                const uint2 a = V.arcs[e0 + j];
-                const uint32_t dw = ld_a32(&S.dist[a.x]);
-                if ((uint64_t)dv + a.y == dw) {
-                    key[j] = ((uint64_t)dw << 32) | a.x;
-                    who[j] = a.x;
-                }
+                const uint32_t dw = dist_final(S, a[j].x);
+                if ((uint64_t)dv + a[j].y == dw) key[j] = ((uint64_t)dw << 32) | a[j].x;
             }
         }
+        ulonglong2 sw[SUCC_REG];
+#pragma unroll
+        for (int j = 0; j < SUCC_REG; ++j) sw[j] = key[j] ? sd_pair(S, a[j].x) : make_ulonglong2(0, 0);
         /* keys are distinct and nonzero for successors (dw >= 1) */
 #pragma unroll
         for (int r = 0; r < SUCC_REG; ++r) {
             uint64_t best = 0;
-            uint32_t bw = 0;
             int bj = -1;
 #pragma unroll
             for (int j = 0; j < SUCC_REG; ++j)
-                if (key[j] > best) { best = key[j]; bw = who[j]; bj = j; }
+                if (key[j] > best) { best = key[j]; bj = j; }
             if (bj < 0) break;
+            ulonglong2 b = make_ulonglong2(0, 0);
 #pragma unroll
             for (int j = 0; j < SUCC_REG; ++j)
-                if (j == bj) key[j] = 0;
-            acc.add(sv_int, sv, ld_a64(&S.sigma[bw]), S.delta[bw]);
+                if (j == bj) { key[j] = 0; b = sw[j]; }
+            acc.add(sv_int, sv, b.x, b.y);
         }
         return acc;
     }
@@ -494,20 +668,60 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot &
         uint64_t best = 0;
         for (uint32_t e = e0; e < e1; ++e) {
             const uint2 a = V.arcs[e];
-            const uint32_t dw = ld_a32(&S.dist[a.x]);
+            const uint32_t dw = dist_final(S, a.x);
             if ((uint64_t)dv + a.y != dw) continue;
             const uint64_t k = ((uint64_t)dw << 32) | a.x;
             if (k < prev && k > best) best = k;
         }
         if (best == 0) break;
-        const uint32_t w = (uint32_t)best;
-        acc.add(sv_int, sv, ld_a64(&S.sigma[w]), S.delta[w]);
+        const ulonglong2 sw = sd_pair(S, (uint32_t)best);
+        acc.add(sv_int, sv, sw.x, sw.y);
         prev = best;
     }
     return acc;
 }
 
-/* ---- the kernel ------------------------------------------------------------ */
+/* Phase 3: dependencies, the levels in reverse, and the mode's output. */
+template <int MODE, int PREC, bool ORDERED, class T>
+__device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const View &V, const Slot S, const T &t,
+                                   uint32_t task, uint32_t c, uint32_t src, uint32_t nlev)
+{
+    const uint32_t r0 = t.rank(), R = t.size();
+    const double wgt = MODE == MODE_PIVOT ? (double)p.task_aux[task] : 1.0;
+    for (int k = (int)nlev - 1; k >= 0; --k) {
+        const uint32_t i0 = S.fa[k], i1 = S.fa[k + 1];
+        for (uint32_t i = i0 + r0; i < i1; i += R) {
+            const uint32_t v = S.qa[i];
+            if (v == src) continue;
+            const Acc<PREC> acc = fold_successors<PREC, ORDERED>(V, S, v);
+            S.sd[2 * (uint64_t)v + 1] = acc.bits();
+            const double dv = acc.value();
+            if (MODE == MODE_DEP) {
+                p.out_dep[(uint64_t)task * V.n + v] = dv;
+            } else if (MODE == MODE_EXACT) {
+                if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
+            } else if (MODE == MODE_PIVOT) {
+                const double x = dv * wgt;
+                if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
+            } else if (MODE == MODE_CLUSTER) {
+                p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
+            }
+        }
+        t.sync();
+        if (t.aborted()) return;
+    }
+}
+
+__device__ __forceinline__ void flush_counters(const BrandesParams &p, Shared &sh, const Counters &ct)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 && sh.te) atomicAdd(p.te, sh.te);
+    if (p.stats)
+        for (int i = 0; i < STAT_COUNT; ++i)
+            if (ct.c[i]) atomicAdd(&p.stats[i], ct.c[i]);
+}
+
+/* ---- the batched kernel: one workgroup per source, device work queue ------- */
 
 template <int MODE, int PREC, bool ORDERED>
 __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
@@ -516,6 +730,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
     const int tid = threadIdx.x;
     Slot S;
     slot_layout(p.nmax, p.scratch + (uint64_t)blockIdx.x * p.slot_bytes, &S);
+    const TeamWG t{&sh.ctl};
     if (tid == 0) sh.te = 0;
     Counters ct;
 
@@ -534,28 +749,34 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
             V.n = p.cl_n[c];
             V.rp = p.rp + ro;
             V.rrp = p.rrp + ro;
-            V.arcs = p.arcs;
-            V.rarcs = p.rarcs;
         } else {
             V.n = p.n;
             V.rp = p.rp;
             V.rrp = p.rrp;
-            V.arcs = p.arcs;
-            V.rarcs = p.rarcs;
         }
+        V.arcs = p.arcs;
+        V.rarcs = p.rarcs;
         const uint32_t src = p.task_src[task];
 
-        phase_distances(V, S, sh, src, p.delta, p.err, ct);
-        const uint32_t nlev = phase_sigma(V, S, sh, src, p.err, ct);
+        const uint64_t t0 = wall_clock64();
+        phase_distances(V, S, sh, t, src, p.delta, p.err, ct);
+        const uint64_t t1 = wall_clock64();
+        const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.err, ct);
+        const uint64_t t2 = wall_clock64();
+        if (tid == 0) {
+            ct.c[STAT_T_DIST] += t1 - t0;
+            ct.c[STAT_T_SIGMA] += t2 - t1;
+        }
 
         if (MODE == MODE_SSSP) {
             uint64_t *od = p.out_dist + (uint64_t)task * V.n;
             uint64_t *os = p.out_sigma + (uint64_t)task * V.n;
             for (uint32_t v = tid; v < V.n; v += WG) {
-                const uint32_t d = ld_a32(&S.dist[v]);
+                const uint32_t d = dist_final(S, v);
                 od[v] = d == DINF ? ~uint64_t(0) : d;
-                os[v] = ld_a64(&S.sigma[v]);
+                os[v] = S.sd[2 * (uint64_t)v];
             }
+            __syncthreads();
             continue;
         }
         if (MODE == MODE_CLUSTER) {
@@ -563,41 +784,47 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
             const uint64_t row = p.cl_vioff[c] + (uint64_t)src * nb;
             for (uint32_t j = tid; j < nb; j += WG) {
                 const uint32_t b = p.cl_blocal[b0 + j];
-                const uint32_t d = ld_a32(&S.dist[b]);
+                const uint32_t d = dist_final(S, b);
                 p.vi_len[row + j] = d == DINF ? 0 : d;
-                p.vi_cnt[row + j] = ld_a64(&S.sigma[b]);
+                p.vi_cnt[row + j] = S.sd[2 * (uint64_t)b];
             }
         }
-
-        /* phase 3: dependencies, levels in reverse */
-        const double wgt = MODE == MODE_PIVOT ? (double)p.task_aux[task] : 1.0;
-        for (int k = (int)nlev - 1; k >= 0; --k) {
-            const uint32_t i0 = S.fa[k], i1 = S.fa[k + 1];
-            for (uint32_t i = i0 + tid; i < i1; i += WG) {
-                const uint32_t v = S.qa[i];
-                if (v == src) continue;
-                const Acc<PREC> acc = fold_successors<PREC, ORDERED>(V, S, v);
-                S.delta[v] = acc.bits();
-                const double dv = acc.value();
-                if (MODE == MODE_DEP) {
-                    p.out_dep[(uint64_t)task * V.n + v] = dv;
-                } else if (MODE == MODE_EXACT) {
-                    if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
-                } else if (MODE == MODE_PIVOT) {
-                    const double x = dv * wgt;
-                    if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
-                } else if (MODE == MODE_CLUSTER) {
-                    p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
-                }
-            }
-            __syncthreads();
-        }
+        phase_dependencies<MODE, PREC, ORDERED>(p, V, S, t, task, c, src, nlev);
+        if (tid == 0) ct.c[STAT_T_DEP] += wall_clock64() - t2;
     }
-    __syncthreads();
-    if (tid == 0 && sh.te) atomicAdd(p.te, sh.te);
-    if (p.stats)
-        for (int i = 0; i < STAT_COUNT; ++i)
-            if (ct.c[i]) atomicAdd(&p.stats[i], ct.c[i]);
+    flush_counters(p, sh, ct);
+}
+
+/* ---- the plain kernel: one source per cooperative launch over the grid ---- */
+
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
+{
+    __shared__ Shared sh;
+    Slot S;
+    slot_layout(p.nmax, p.scratch, &S);
+    const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl)};
+    if (threadIdx.x == 0) sh.te = 0;
+    Counters ct;
+    View V;
+    V.n = p.n;
+    V.rp = p.rp;
+    V.rrp = p.rrp;
+    V.arcs = p.arcs;
+    V.rarcs = p.rarcs;
+    const uint32_t src = p.task_src[task];
+    const uint64_t t0 = wall_clock64();
+    phase_distances(V, S, sh, t, src, p.delta, p.err, ct);
+    const uint64_t t1 = wall_clock64();
+    const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.err, ct);
+    const uint64_t t2 = wall_clock64();
+    phase_dependencies<MODE, 0, true>(p, V, S, t, task, 0, src, nlev);
+    if (t.leader()) {
+        ct.c[STAT_T_DIST] += t1 - t0;
+        ct.c[STAT_T_SIGMA] += t2 - t1;
+        ct.c[STAT_T_DEP] += wall_clock64() - t2;
+    }
+    flush_counters(p, sh, ct);
 }
 
 template <int MODE, int PREC, bool ORDERED>
@@ -627,6 +854,37 @@ hipError_t launch_brandes(int mode, int prec, bool ordered, const BrandesParams 
     case MODE_CLUSTER: return launch_one<MODE_CLUSTER, 0, true>(p, slots, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+static const void *plain_kernel(int mode)
+{
+    switch (mode) {
+    case MODE_DEP: return reinterpret_cast<const void *>(&k_plain<MODE_DEP>);
+    case MODE_PIVOT: return reinterpret_cast<const void *>(&k_plain<MODE_PIVOT>);
+    default: return nullptr;
+    }
+}
+
+hipError_t plain_max_blocks(int mode, int cu_count, uint32_t *blocks)
+{
+    const void *f = plain_kernel(mode);
+    if (!f) return hipErrorInvalidValue;
+    int per_cu = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, 0);
+    if (e != hipSuccess) return e;
+    *blocks = (uint32_t)(per_cu * cu_count);
+    return hipSuccess;
+}
+
+hipError_t launch_plain(int mode, const BrandesParams &p, uint32_t task, uint32_t blocks,
+                        hipStream_t stream)
+{
+    const void *f = plain_kernel(mode);
+    if (!f) return hipErrorInvalidValue;
+    BrandesParams pc = p;
+    uint32_t tk = task;
+    void *args[] = {&pc, &tk};
+    return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(WG), args, 0, stream);
 }
 
 /* ---- cluster reduction and pivot subtraction ------------------------------- */

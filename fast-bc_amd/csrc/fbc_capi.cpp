@@ -32,8 +32,10 @@ struct fbc_ctx {
     size_t scratch_bytes = 0;
     uint32_t *d_queue = nullptr, *d_err = nullptr;
     unsigned long long *d_te = nullptr, *d_stats = nullptr;
+    uint32_t *d_grid_ctl = nullptr;   /* plain kernel: grid-shared traversal counters */
     double last_ms = 0, last_te = 0;
     unsigned long long last_counters[fbcdev::STAT_COUNT] = {};
+    double wall_khz = 100000.0;   /* device wall clock (wall_clock64) rate */
 };
 
 struct fbc_graph {
@@ -148,8 +150,10 @@ void base_params(BrandesParams &p, const fbc_graph *g)
     p.nmax = g->n;
 }
 
-/* Launch one persistent run over p.ntasks tasks and wait for it. */
-int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p)
+/* Launch one run over p.ntasks tasks and wait for it: the persistent batched
+ * kernel, or (plain) one cooperative grid launch per task. */
+int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p,
+        bool plain = false)
 {
     if (p.ntasks == 0) {
         ctx->last_ms = 0;
@@ -157,7 +161,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
         return 0;
     }
     uint32_t slots = 0;
-    int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots);
+    int rc = ensure_scratch(ctx, p.nmax, plain ? 1 : p.ntasks, &slots);
     if (rc) return rc;
     p.scratch = ctx->scratch;
     p.slot_bytes = fbcdev::slot_bytes(p.nmax);
@@ -171,7 +175,17 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_te, 0, sizeof(unsigned long long), ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, p, slots, ctx->stream));
+    if (plain) {
+        uint32_t blocks = 0;
+        HIP_TRY(ctx, fbcdev::plain_max_blocks(mode, ctx->cu_count, &blocks));
+        const uint32_t want = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count;
+        if (blocks > want) blocks = want;
+        p.grid_ctl = ctx->d_grid_ctl;
+        for (uint32_t t = 0; t < p.ntasks; ++t)
+            HIP_TRY(ctx, fbcdev::launch_plain(mode, p, t, blocks, ctx->stream));
+    } else {
+        HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, p, slots, ctx->stream));
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
@@ -181,6 +195,8 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(err), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(ctx->last_counters, ctx->d_stats, sizeof(ctx->last_counters), hipMemcpyDeviceToHost));
+    for (int i = fbcdev::STAT_T_DIST; i <= fbcdev::STAT_T_DEP; ++i)   /* wall-clock ticks -> ns */
+        ctx->last_counters[i] = (unsigned long long)((double)ctx->last_counters[i] * 1e6 / ctx->wall_khz);
     ctx->last_ms = ms;
     ctx->last_te = (double)te;
     if (err & fbcdev::ERR_SIGMA)
@@ -209,10 +225,14 @@ int fbc_create(int device, fbc_ctx **out)
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
     ctx->total_mem = prop.totalGlobalMem;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
+        ctx->wall_khz = wall_khz;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         dalloc(&ctx->d_queue, 1) != hipSuccess || dalloc(&ctx->d_err, 1) != hipSuccess ||
-        dalloc(&ctx->d_te, 1) != hipSuccess || dalloc(&ctx->d_stats, fbcdev::STAT_COUNT) != hipSuccess) {
+        dalloc(&ctx->d_te, 1) != hipSuccess || dalloc(&ctx->d_stats, fbcdev::STAT_COUNT) != hipSuccess ||
+        dalloc(&ctx->d_grid_ctl, 16) != hipSuccess || hipMemset(ctx->d_grid_ctl, 0, 16 * sizeof(uint32_t)) != hipSuccess) {
         fbc_destroy(ctx);
         return FBC_E_HIP;
     }
@@ -228,6 +248,7 @@ void fbc_destroy(fbc_ctx *ctx)
     dfree(ctx->d_err);
     dfree(ctx->d_te);
     dfree(ctx->d_stats);
+    dfree(ctx->d_grid_ctl);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -324,7 +345,7 @@ void fbc_graph_free(fbc_graph *g)
 
 /* Output-row modes (sssp / dep): sources in chunks that fit a device budget. */
 static int row_mode(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, const uint32_t *src,
-                    uint32_t nsrc, uint64_t *dist, uint64_t *sigma, double *dep)
+                    uint32_t nsrc, uint64_t *dist, uint64_t *sigma, double *dep, bool plain = false)
 {
     for (uint32_t i = 0; i < nsrc; ++i)
         if (src[i] >= g->n) return fail(ctx, FBC_E_ARG, "source out of range");
@@ -360,7 +381,7 @@ static int row_mode(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, const 
             rc = hip_fail(ctx, e, "chunk setup");
             break;
         }
-        rc = run(ctx, g, mode, prec, true, p);
+        rc = run(ctx, g, mode, prec, true, p, plain);
         if (rc) break;
         ms += ctx->last_ms;
         te += ctx->last_te;
@@ -399,7 +420,10 @@ int fbc_dep_batch(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_
 
 int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec, double *dep)
 {
-    return fbc_dep_batch(ctx, g, &src, 1, prec, dep);
+    if (!ctx || !g || !dep) return fail(ctx, FBC_E_ARG, "null argument");
+    if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    if (prec == FBC_PREC_F32C) return fbc_dep_batch(ctx, g, &src, 1, prec, dep);
+    return row_mode(ctx, g, fbcdev::MODE_DEP, prec, &src, 1, nullptr, nullptr, dep, true);
 }
 
 int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc)
@@ -686,9 +710,9 @@ int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uin
     return rc;
 }
 
-int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot, const uint32_t *weight,
-                    const uint32_t *pivot_cluster, uint32_t npiv, const double *intra_bc, int prec,
-                    double *bc)
+static int pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot, const uint32_t *weight,
+                       const uint32_t *pivot_cluster, uint32_t npiv, const double *intra_bc, int prec,
+                       double *bc, bool plain)
 {
     if (!ctx || !cl || !bc || !intra_bc || (npiv && (!pivot || !weight || !pivot_cluster)))
         return fail(ctx, FBC_E_ARG, "null argument");
@@ -722,7 +746,7 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
         p.task_src = d_tsrc;
         p.task_aux = d_taux;
         p.bc = d_bc;
-        rc = run(ctx, g, fbcdev::MODE_PIVOT, prec, false, p);
+        rc = run(ctx, g, fbcdev::MODE_PIVOT, prec, false, p, plain);
     }
     if (!rc) {
         e = fbcdev::launch_pivot_subtract(cl->gid.size(), cl->d_gid, cl->d_vcl, d_cptr, d_pw, d_intra, d_bc,
@@ -738,6 +762,19 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
     dfree(d_cptr);
     dfree(d_pw);
     return rc;
+}
+
+int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot, const uint32_t *weight,
+                    const uint32_t *pivot_cluster, uint32_t npiv, const double *intra_bc, int prec,
+                    double *bc)
+{
+    return pivot_phase(ctx, cl, pivot, weight, pivot_cluster, npiv, intra_bc, prec, bc, false);
+}
+
+int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot, const uint32_t *weight,
+                          const uint32_t *pivot_cluster, uint32_t npiv, const double *intra_bc, double *bc)
+{
+    return pivot_phase(ctx, cl, pivot, weight, pivot_cluster, npiv, intra_bc, FBC_PREC_F64, bc, true);
 }
 
 int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,

@@ -66,6 +66,7 @@ struct BrandesParams {
     uint32_t *err;
     unsigned long long *te;   /* traversed-edge counter */
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
+    uint32_t *grid_ctl;       /* plain kernel: traversal counters shared by the grid (16 words) */
 };
 
 /* Work counters of the persistent kernel (fbc_last_stats_ex). */
@@ -77,11 +78,21 @@ enum Stat : int {
     STAT_IMPROVE = 4,     /* relaxations that lowered a distance */
     STAT_FARPUSH = 5,     /* far-list insertions */
     STAT_LEVELS = 6,      /* levels of the shortest-path DAG */
-    STAT_COUNT = 8
+    STAT_T_DIST = 7,      /* time in the distance phase, summed over sources */
+    STAT_T_SIGMA = 8,     /* time in the sigma phase */
+    STAT_T_DEP = 9,       /* time in the dependency phase */
+    STAT_COUNT = 10
 };
 
 /* Bytes of one slot for views of at most nmax vertices. */
 uint64_t slot_bytes(uint32_t nmax);
+
+/* Plain form: one source (task index 'task') per cooperative launch of
+ * 'blocks' workgroups; modes MODE_DEP and MODE_PIVOT, FP64 in the reference's
+ * successor order.  plain_max_blocks gives the co-resident limit. */
+hipError_t plain_max_blocks(int mode, int cu_count, uint32_t *blocks);
+hipError_t launch_plain(int mode, const BrandesParams &p, uint32_t task, uint32_t blocks,
+                        hipStream_t stream);
 
 /* Launch the persistent kernel (grid = slots workgroups) on 'stream'. */
 hipError_t launch_brandes(int mode, int prec, bool ordered, const BrandesParams &p,

@@ -24,13 +24,14 @@ EXPORTS = (
     "fbc_graph_free", "fbc_sssp_sigma", "fbc_single_source_dep", "fbc_dep_batch",
     "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
-    "fbc_last_counters", "fbc_kmeans_batch",
+    "fbc_last_counters", "fbc_kmeans_batch", "fbc_pivot_phase_plain",
 )
 
 # Every symbol include/fbc_host.h declares.
 HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_exact_bc_arcs")
 
-COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels")
+COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
+                 "ns_dist", "ns_sigma", "ns_dep")
 
 ERRORS = {
     -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",
@@ -81,6 +82,7 @@ def lib():
         L.fbc_cluster_eval.argtypes = [_vp, _vp, _f64p, _u64p, _u64p]
         L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
                                       ctypes.c_int, _f64p]
+        L.fbc_pivot_phase_plain.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
         L.fbc_kmeans_batch.argtypes = [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _f64p,
@@ -139,8 +141,8 @@ class Context:
         return ms.value, te.value
 
     def last_counters(self):
-        c = np.zeros(8, np.uint64)
-        lib().fbc_last_counters(self.h, c, 8)
+        c = np.zeros(len(COUNTER_NAMES), np.uint64)
+        lib().fbc_last_counters(self.h, c, len(COUNTER_NAMES))
         return dict(zip(COUNTER_NAMES, (int(x) for x in c)))
 
     def graph(self, n, row_ptr, col, w):
@@ -295,12 +297,19 @@ class Clusters:
         nc = int(self.ptr[c + 1] - self.ptr[c])
         return flat[self.vi_off[c]:self.vi_off[c] + nc * nb].reshape(nc, nb)
 
-    def pivot_phase(self, pivots, weights, pivot_cluster, intra_bc, bc, prec=PREC_F64):
+    def pivot_phase(self, pivots, weights, pivot_cluster, intra_bc, bc, prec=PREC_F64, plain=False):
+        """bc + the pivot loop's terms.  plain=True: one whole-GPU launch per
+        pivot, FP64 (the baseline form); otherwise all pivots in one launch."""
         out = np.ascontiguousarray(bc, np.float64).copy()
         p = _u32(pivots)
-        self.graph.ctx.check(lib().fbc_pivot_phase(
-            self.graph.ctx.h, self.h, p, _u32(weights), _u32(pivot_cluster), len(p),
-            np.ascontiguousarray(intra_bc, np.float64), prec, out))
+        args = (self.graph.ctx.h, self.h, p, _u32(weights), _u32(pivot_cluster), len(p),
+                np.ascontiguousarray(intra_bc, np.float64))
+        if plain:
+            if prec != PREC_F64:
+                raise ValueError("the plain form computes in FP64")
+            self.graph.ctx.check(lib().fbc_pivot_phase_plain(*args, out))
+        else:
+            self.graph.ctx.check(lib().fbc_pivot_phase(*args, prec, out))
         return out
 
 

@@ -14,6 +14,7 @@
  *       -> fbc_clusters_create + fbc_cluster_eval (all clusters in one launch)
  *   ClusteredBrandeBC pivot loop           brandes/ClusteredBrandesBC.h:141-166
  *       -> fbc_pivot_phase (all pivots in one launch)
+ *       -> fbc_pivot_phase_plain (one launch per pivot, the baseline form)
  *
  * Conventions
  *   - Return 0 on success or a negative FBC_E_* status; fbc_last_error()
@@ -84,7 +85,11 @@ int fbc_sssp_sigma(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32
 
 /* ISSBrandesBC::singleSourceBrandes (brandes/DijkstraSSBrandesBC.h:45-81):
  * dep[n] = dependency of every vertex on paths from src (0 for src and for
- * unreached vertices).  One source per call: the plain form. */
+ * unreached vertices).  One source per call: the plain form.  With
+ * FBC_PREC_F64 the source is spread over the whole GPU (one cooperative
+ * launch, grid-wide barriers) and the sums follow the reference's successor
+ * order, so dep equals the reference's bit for bit; FBC_PREC_F32C runs the
+ * batched kernel with a single source. */
 int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec,
                           double *dep);
 
@@ -132,6 +137,17 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
                     const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
                     const double *intra_bc, int prec, double *bc);
 
+/* The same pivot loop in the plain form, as the reference runs it: one
+ * singleSourceBrandes per pivot (ClusteredBrandesBC.h:143-148 calling
+ * DijkstraSSBrandesBC.h:45-81), here one cooperative whole-GPU launch per
+ * pivot with FP64 sums in the reference's successor order.  Same result
+ * contract as fbc_pivot_phase with FBC_PREC_F64; the baseline the batched
+ * form is measured against.  fbc_set_tuning's 'slots' sets the workgroups of
+ * the grid (0: one per CU). */
+int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
+                          const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
+                          const double *intra_bc, double *bc);
+
 /* PlusPlusKMeans::computeCentroids (kmeans/PlusPlusKMeans.h:70-229) for nsets
  * independent point sets in one batch (one set per cluster's exact pivots, as
  * KMeansPivotSelector calls it, brandes/KMeansPivotSelector.h:99-101).
@@ -150,10 +166,12 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
  * and the number of forward arcs it examined (traversed edges). */
 int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edges);
 
-/* Work counters of the last launch (distance-phase rounds, windows, expanded
- * vertices, relaxations, improvements, far insertions, DAG levels, summed over
- * its sources).  Fills at most 'count' entries, returns how many it wrote.
- * Instrumentation only; no counterpart in the reference. */
+/* Work counters of the last launch, summed over its sources: distance-phase
+ * rounds, windows, expanded vertices, relaxations, improvements, far
+ * insertions, DAG levels, then the nanoseconds spent in the distance, sigma and
+ * dependency phases (per workgroup, so the sum exceeds the kernel time by the
+ * number of workgroups).  Fills at most 'count' entries, returns how many it
+ * wrote.  Instrumentation only; no counterpart in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
 
 #ifdef __cplusplus

@@ -138,6 +138,43 @@ def test_cfg0_pivot_phase(cfg0):
     assert (err <= 1e-9 * np.abs(ref) + 1e-13 * mag).all(), (err / np.maximum(mag, 1)).max()
 
 
+def test_cfg0_plain_single_source_bitexact(cfg0):
+    """The plain form (one whole-GPU cooperative launch per source, FP64 in
+    the reference's successor order) equals the oracle bit for bit."""
+    g, og = cfg0
+    for s in (0, 17, 2049, g.n - 1):
+        assert np.array_equal(g.single_source_dep(s), og.single_source_dep(s)), f"source {s}"
+
+
+def test_rmat_plain_single_source_heavy_rows(gpu_ctx):
+    """R-MAT hubs exceed the per-thread degree limit, so their arcs go through
+    the workgroup-split path of every phase."""
+    n, (s, d, w) = F.generate_rmat(10, 16, seed=11)
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    deg = np.bincount(s, minlength=n)
+    assert deg.max() > 96
+    for src in (int(deg.argmax()), 0, n - 1):
+        assert np.array_equal(g.single_source_dep(src), og.single_source_dep(src)), f"source {src}"
+
+
+def test_cfg0_pivot_phase_plain_matches_batched(cfg0):
+    g, og = cfg0
+    r = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=0.5, threads=1)
+    cl = g.clusters(comm_of=r["comm"])
+    args = (r["pivots"], r["weights"], r["pivot_cluster"], r["intra"], r["intra"])
+    plain = cl.pivot_phase(*args, plain=True)
+    batched = cl.pivot_phase(*args)
+    ref = r["bc"]
+    dep = g.dep_batch(r["pivots"])
+    mag = np.abs(r["intra"]) + (np.abs(dep) * r["weights"][:, None]).sum(0)
+    wsum = np.zeros(cl.ncl)
+    np.add.at(wsum, r["pivot_cluster"], r["weights"])
+    mag += wsum[r["comm"]] * np.abs(r["intra"])
+    for out in (plain, batched):
+        err = np.abs(out - ref)
+        assert (err <= 1e-9 * np.abs(ref) + 1e-13 * mag).all(), (err / np.maximum(mag, 1)).max()
+
+
 def interleave(L, C):
     """VertexInfo rows as k-means features: length then count per border."""
     L, C = np.asarray(L, float), np.asarray(C, float)
