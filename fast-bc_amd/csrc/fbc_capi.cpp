@@ -101,6 +101,16 @@ hipError_t upload(T **p, const std::vector<T> &h, hipStream_t s)
 }
 
 template <typename T>
+hipError_t upload_n(T **p, const T *h, size_t n, hipStream_t s)
+{
+    hipError_t e = dalloc(p, n);
+    if (e != hipSuccess || !n) return e;
+    e = hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+template <typename T>
 void dfree(T *&p)
 {
     if (p) (void)hipFree((void *)p);
@@ -784,10 +794,11 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     if (!ctx || (nsets && (!npoints || !nfeat || !k || !features || !weights || !centroids || !cweights)))
         return fail(ctx, FBC_E_ARG, "null argument");
     if (nsets == 0) return 0;
-    std::vector<uint64_t> xoff(nsets), poff(nsets), koff(nsets), moff(nsets);
+    std::vector<uint64_t> xoff(nsets), poff(nsets), koff(nsets), moff(nsets), goff(nsets, 0);
     uint64_t xs = 0, ps = 0, ks = 0, ms = 0;
     for (uint32_t s = 0; s < nsets; ++s) {
         if (npoints[s] == 0 || k[s] == 0 || k[s] > npoints[s]) return fail(ctx, FBC_E_ARG, "bad k-means set sizes");
+        if (k[s] > FBC_KMEANS_MAX_K) return fail(ctx, FBC_E_UNSUPPORTED, "k-means set with too many centroids");
         xoff[s] = xs;
         poff[s] = ps;
         koff[s] = ks;
@@ -797,41 +808,47 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         ks += k[s];
         ms += (uint64_t)k[s] * nfeat[s];
     }
-    /* exactness precondition and the exact squared norms */
-    std::vector<double> xnorm(ps);
+    /* exactness precondition (the squared norms come from the Gram diagonal) */
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
     for (uint32_t s = 0; s < nsets; ++s) {
         double mx = 0;
         const double *X = features + xoff[s];
         for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) {
             const double x = X[i];
-            if (x != std::floor(x)) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features must be integers");
+            if (x != std::floor(x)) bad |= 1;
             mx = std::max(mx, std::fabs(x));
         }
-        if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0)
-            return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
-        for (uint32_t v = 0; v < npoints[s]; ++v) {
-            double q = 0;
-            for (uint32_t f = 0; f < nfeat[s]; ++f) q += X[(uint64_t)v * nfeat[s] + f] * X[(uint64_t)v * nfeat[s] + f];
-            xnorm[poff[s] + v] = q;
+        if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0) bad |= 2;
+    }
+    if (bad & 1) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features must be integers");
+    if (bad & 2) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
+    /* groups of sets whose Gram matrices fit the budget together */
+    const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30), 1);
+    std::vector<std::vector<uint32_t>> groups;
+    uint64_t gmax = 0;
+    {
+        uint64_t used = 0;
+        for (uint32_t s = 0; s < nsets; ++s) {
+            const uint64_t g = (uint64_t)npoints[s] * npoints[s];
+            if (groups.empty() || (used + g) * 8 > budget) {
+                groups.emplace_back();
+                used = 0;
+            }
+            goff[s] = used;
+            used += g;
+            groups.back().push_back(s);
+            gmax = std::max(gmax, used);
         }
     }
-    std::vector<uint32_t> tcl, tp0;
-    for (uint32_t s = 0; s < nsets; ++s)
-        for (uint32_t p0 = 0; p0 < npoints[s]; p0 += 64) {
-            tcl.push_back(s);
-            tp0.push_back(p0);
-        }
-    std::vector<uint32_t> P(npoints, npoints + nsets), D(nfeat, nfeat + nsets), K(k, k + nsets),
-        active(nsets, 1);
-    std::vector<double> X(features, features + xs);
-    std::vector<int32_t> W(weights, weights + ps);
+    std::vector<uint32_t> P(npoints, npoints + nsets), D(nfeat, nfeat + nsets), K(k, k + nsets), active(nsets, 0);
     fbcdev::KmParams p;
     std::memset(&p, 0, sizeof(p));
     p.ncl = nsets;
     p.min_variance = min_variance;
-    uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtcl = nullptr, *dtp0 = nullptr;
-    uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr;
-    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmean = nullptr;
+    uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtiles = nullptr, *dtcl = nullptr, *dtp0 = nullptr;
+    uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr, *dgo = nullptr;
+    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmean = nullptr, *dG = nullptr;
     int32_t *dW = nullptr, *dcw = nullptr;
     uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
              *dmem = nullptr, *dact = nullptr;
@@ -841,14 +858,15 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     if ((e = upload(&dP, P, ctx->stream)) != hipSuccess || (e = upload(&dD, D, ctx->stream)) != hipSuccess ||
         (e = upload(&dK, K, ctx->stream)) != hipSuccess || (e = upload(&dxo, xoff, ctx->stream)) != hipSuccess ||
         (e = upload(&dpo, poff, ctx->stream)) != hipSuccess || (e = upload(&dko, koff, ctx->stream)) != hipSuccess ||
-        (e = upload(&dmo, moff, ctx->stream)) != hipSuccess || (e = upload(&dX, X, ctx->stream)) != hipSuccess ||
-        (e = upload(&dxn, xnorm, ctx->stream)) != hipSuccess || (e = upload(&dW, W, ctx->stream)) != hipSuccess ||
-        (e = upload(&dtcl, tcl, ctx->stream)) != hipSuccess || (e = upload(&dtp0, tp0, ctx->stream)) != hipSuccess ||
-        (e = upload(&dact, active, ctx->stream)) != hipSuccess || (e = dalloc(&dcd, ps)) != hipSuccess ||
-        (e = dalloc(&dmean, ms)) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
+        (e = upload(&dmo, moff, ctx->stream)) != hipSuccess || (e = upload(&dgo, goff, ctx->stream)) != hipSuccess ||
+        (e = upload_n(&dX, features, xs, ctx->stream)) != hipSuccess || (e = dalloc(&dxn, ps)) != hipSuccess ||
+        (e = upload_n(&dW, weights, ps, ctx->stream)) != hipSuccess || (e = upload(&dact, active, ctx->stream)) != hipSuccess ||
+        (e = dalloc(&dcd, ps)) != hipSuccess || (e = dalloc(&dmd, ps)) != hipSuccess ||
+        (e = dalloc(&dmean, std::max<uint64_t>(ms, 1))) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
         (e = dalloc(&dcent, ks)) != hipSuccess || (e = dalloc(&dnext, ks)) != hipSuccess ||
         (e = dalloc(&dcnt, ks)) != hipSuccess || (e = dalloc(&dstart, ks)) != hipSuccess ||
-        (e = dalloc(&dasg, ps)) != hipSuccess || (e = dalloc(&dmem, ps)) != hipSuccess)
+        (e = dalloc(&dasg, ps)) != hipSuccess || (e = dalloc(&dmem, ps)) != hipSuccess ||
+        (e = dalloc(&dG, gmax)) != hipSuccess)
         rc = hip_fail(ctx, e, "k-means setup");
     if (!rc) {
         p.P = dP;
@@ -858,10 +876,13 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         p.poff = dpo;
         p.koff = dko;
         p.moff = dmo;
+        p.goff = dgo;
         p.X = dX;
         p.xnorm = dxn;
+        p.G = dG;
         p.weights = dW;
         p.cdist = dcd;
+        p.mdist = dmd;
         p.cent = dcent;
         p.next = dnext;
         p.cnt = dcnt;
@@ -872,15 +893,44 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         p.cweights = dcw;
         p.active = dact;
         (void)hipEventRecord(ctx->ev0, ctx->stream);
-        e = fbcdev::km_launch_init(p, ctx->stream);
-        for (uint32_t it = 1; e == hipSuccess && it <= max_iteration + 1; ++it) {
-            e = fbcdev::km_launch_iteration(p, (uint32_t)tcl.size(), dtcl, dtp0, it, max_iteration, ctx->stream);
+        for (const std::vector<uint32_t> &grp : groups) {
+            std::vector<uint32_t> tiles, tcl, tp0;
+            std::fill(active.begin(), active.end(), 0u);
+            for (uint32_t s : grp) {
+                active[s] = 1;
+                const uint32_t nt = (P[s] + 63) / 64;
+                for (uint32_t ti = 0; ti < nt; ++ti)
+                    for (uint32_t tj = ti; tj < nt; ++tj) {
+                        tiles.push_back(s);
+                        tiles.push_back(ti);
+                        tiles.push_back(tj);
+                    }
+                for (uint32_t p0 = 0; p0 < P[s]; p0 += 256) {
+                    tcl.push_back(s);
+                    tp0.push_back(p0);
+                }
+            }
+            dfree(dtiles);
+            dfree(dtcl);
+            dfree(dtp0);
+            if ((e = upload(&dtiles, tiles, ctx->stream)) != hipSuccess || (e = upload(&dtcl, tcl, ctx->stream)) != hipSuccess ||
+                (e = upload(&dtp0, tp0, ctx->stream)) != hipSuccess ||
+                (e = hipMemcpyAsync(dact, active.data(), nsets * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                    ctx->stream)) != hipSuccess ||
+                (e = fbcdev::km_launch_gram(p, dG, (uint32_t)(tiles.size() / 3), dtiles, ctx->stream)) != hipSuccess ||
+                (e = fbcdev::km_launch_init(p, ctx->stream)) != hipSuccess)
+                break;
+            for (uint32_t it = 1; it <= max_iteration + 1; ++it) {
+                e = fbcdev::km_launch_iteration(p, (uint32_t)tcl.size(), dtcl, dtp0, it, max_iteration, ctx->stream);
+                if (e != hipSuccess) break;
+                e = hipMemcpyAsync(active.data(), dact, nsets * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+                if (e != hipSuccess) break;
+                bool any = false;
+                for (uint32_t a : active) any = any || a;
+                if (!any) break;
+            }
             if (e != hipSuccess) break;
-            e = hipMemcpyAsync(active.data(), dact, nsets * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            bool any = false;
-            for (uint32_t a : active) any = any || a;
-            if (!any) break;
         }
         (void)hipEventRecord(ctx->ev1, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -891,9 +941,9 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         if (e == hipSuccess) e = hipMemcpy(cweights, dcw, ks * sizeof(int32_t), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "k-means run");
     }
-    for (auto *q : {dP, dD, dK, dtcl, dtp0, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
-    for (auto *q : {dxo, dpo, dko, dmo}) dfree(q);
-    for (auto *q : {dX, dxn, dcd, dmean}) dfree(q);
+    for (auto *q : {dP, dD, dK, dtiles, dtcl, dtp0, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
+    for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
+    for (auto *q : {dX, dxn, dcd, dmd, dmean, dG}) dfree(q);
     dfree(dW);
     dfree(dcw);
     ctx->last_ms = ms_total;

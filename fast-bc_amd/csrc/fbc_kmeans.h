@@ -8,7 +8,8 @@
 namespace fbcdev {
 
 /* A batch of independent k-means problems (one per cluster); all arrays are
- * concatenations indexed through the per-set offsets. */
+ * concatenations indexed through the per-set offsets.  Sets whose active flag
+ * is 0 are skipped by every kernel. */
 struct KmParams {
     uint32_t ncl;
     const uint32_t *P, *D, *K;          /* points, features, centroids per set */
@@ -16,10 +17,13 @@ struct KmParams {
     const uint64_t *poff;               /* per-point arrays */
     const uint64_t *koff;               /* per-centroid arrays */
     const uint64_t *moff;               /* mean block (K * D doubles) */
+    const uint64_t *goff;               /* Gram block of the set in G (P * P doubles) */
     const double *X;                    /* features, row-major */
-    const double *xnorm;                /* squared norm of every row (exact) */
+    double *xnorm;                      /* squared norm of every row (exact; the Gram diagonal) */
+    const double *G;                    /* Gram matrices X X^T of the active sets (exact) */
     const int32_t *weights;             /* per point */
     double *cdist;                      /* per point, initialisation */
+    double *mdist;                      /* per member slot: distance to its centroid's mean */
     uint32_t *cent, *next, *cnt, *start;/* per centroid */
     uint32_t *assign, *members;         /* per point */
     double *mean;
@@ -28,7 +32,15 @@ struct KmParams {
     double min_variance;
 };
 
+/* Largest centroid count whose member bookkeeping fits the update kernel's LDS. */
+constexpr uint32_t KM_MAX_K = 12288;
+
+/* Gram tiles: tile t covers rows [64 * ti, +64) x [64 * tj, +64) of set s,
+ * tj >= ti, packed as (s, ti, tj) triples. */
+hipError_t km_launch_gram(const KmParams &p, double *G, uint32_t ntiles, const uint32_t *tiles,
+                          hipStream_t s);
 hipError_t km_launch_init(const KmParams &p, hipStream_t s);
+/* point tiles: (set, first point) pairs of 256 points */
 hipError_t km_launch_iteration(const KmParams &p, uint32_t ntiles, const uint32_t *tile_cl,
                                const uint32_t *tile_p0, uint32_t iteration, uint32_t max_iteration,
                                hipStream_t s);

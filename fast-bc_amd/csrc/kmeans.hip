@@ -5,29 +5,37 @@
  *
  * Points are VertexInfo rows: feature 2j = path length to border j, 2j+1 =
  * path count (VertexInfo.h:293-307 sums, per border, the length term then the
- * count term; the feature order keeps that order).  Point-to-point distances
- * are exact integers when every |feature| is small enough (checked by the
- * host: 2 * D * max^2 < 2^53), so the assignment step ranks centroids by
- * ||c||^2 - 2 c.x, an exact integer in FP64 whatever the summation order, and
- * ties go to the first centroid as in the reference.  Mean-to-member
- * distances are not integers; they are summed in the reference's sequential
- * order by one thread each.
+ * count term; the feature order keeps that order).  Every feature is an
+ * integer and the host checks 2 * D * max|x|^2 < 2^53, so the Gram matrix
+ * G = X X^T and every point-to-point distance ||x_a||^2 + ||x_b||^2 - 2 G[a][b]
+ * are exact integers in FP64 whatever the summation order.  G is computed once
+ * per set; the farthest-point initialisation, the assignment and the
+ * variance test then read distances from it (one coalesced row per centroid)
+ * instead of recomputing dot products in every step.  Ties go to the first
+ * point / centroid as in the reference.  Mean-to-member distances are not
+ * integers; each is summed in the reference's sequential feature order by one
+ * thread, and the nearest member is the first strict minimum in point order.
  *
- * Kernels (one workgroup per cluster unless noted):
- *   k_km_init     farthest-point initialisation (PlusPlusKMeans.h:172-207)
- *   k_km_assign   tiled point x centroid dot products + first-wins argmin
- *                 (one workgroup per 64-point tile)
- *   k_km_update   members, means, nearest member to each mean, the variance
- *                 test (PlusPlusKMeans.h:115-151) and the final weights
+ * Kernels:
+ *   k_km_gram    64 x 64 tiles of G (upper triangle, mirrored)
+ *   k_km_init    farthest-point initialisation (PlusPlusKMeans.h:172-207),
+ *                one 1024-thread workgroup per set
+ *   k_km_assign  first-wins nearest centroid of every point, 256 points per
+ *                workgroup
+ *   k_km_update  members, means, nearest member to each mean, the variance
+ *                test (PlusPlusKMeans.h:115-151) and the final weights, one
+ *                1024-thread workgroup per set
  */
 #include "fbc_kmeans.h"
 
 namespace fbcdev {
 
-constexpr int KWG = 256;
-constexpr int TP = 64;      /* points per assignment tile */
-constexpr int TC = 64;      /* centroids per inner tile */
-constexpr int TD = 16;      /* features per LDS stage */
+constexpr int GWG = 256;    /* Gram tile workgroup */
+constexpr int GT = 64;      /* Gram tile edge */
+constexpr int GD = 16;      /* features per LDS stage */
+constexpr int SWG = 1024;   /* per-set workgroups */
+constexpr int NWAVE = SWG / 64;
+constexpr int AWG = 256;    /* points per assignment workgroup */
 
 /* sum over features of (a - b)^2 in feature order */
 __device__ __forceinline__ double sq_rows(const double *a, const double *b, uint32_t D)
@@ -40,26 +48,119 @@ __device__ __forceinline__ double sq_rows(const double *a, const double *b, uint
     return sq;
 }
 
-__global__ __launch_bounds__(KWG) void k_km_init(KmParams p)
+/* exact squared distance between points a and b of a set */
+__device__ __forceinline__ double gram_dist(const double *Gs, const double *xn, uint32_t P, uint32_t a,
+                                            uint32_t b)
+{
+    return (xn[a] + xn[b]) - 2.0 * Gs[(uint64_t)a * P + b];
+}
+
+/* Exclusive prefix sum over the SWG threads of a workgroup; returns the
+ * thread's offset, *total the sum. */
+__device__ uint32_t block_scan(uint32_t x, uint32_t *s_wave, uint32_t *total)
+{
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    uint32_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < NWAVE ? s_wave[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (lane < NWAVE) s_wave[lane] = w;   /* inclusive wave totals */
+    }
+    __syncthreads();
+    const uint32_t base = wave ? s_wave[wave - 1] : 0;
+    *total = s_wave[NWAVE - 1];
+    __syncthreads();
+    return base + inc - x;
+}
+
+/* grid: one workgroup per tile (s, ti, tj), tj >= ti */
+__global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const uint32_t *tiles)
+{
+    const uint32_t s = tiles[3 * blockIdx.x], ti = tiles[3 * blockIdx.x + 1], tj = tiles[3 * blockIdx.x + 2];
+    const uint32_t P = p.P[s], D = p.D[s];
+    const double *X = p.X + p.xoff[s];
+    double *Gs = G + p.goff[s];
+    __shared__ double sa[GD][GT + 1];
+    __shared__ double sb[GD][GT + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const uint32_t r0 = ti * GT, c0 = tj * GT;
+    double acc[4][4] = {};
+    for (uint32_t d0 = 0; d0 < D; d0 += GD) {
+        for (int e = threadIdx.x; e < GD * GT; e += GWG) {
+            const int dd = e % GD, rr = e / GD;
+            const uint32_t f = d0 + dd, ra = r0 + rr, rb = c0 + rr;
+            sa[dd][rr] = (ra < P && f < D) ? X[(uint64_t)ra * D + f] : 0.0;
+            sb[dd][rr] = (rb < P && f < D) ? X[(uint64_t)rb * D + f] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int dd = 0; dd < GD; ++dd) {
+            double xv[4], cv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                xv[a] = sa[dd][ty * 4 + a];
+                cv[a] = sb[dd][tx * 4 + a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fma(xv[a], cv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t r = r0 + ty * 4 + a, c = c0 + tx * 4 + b;
+            if (r < P && c < P) {
+                Gs[(uint64_t)r * P + c] = acc[a][b];
+                Gs[(uint64_t)c * P + r] = acc[a][b];
+                if (r == c) p.xnorm[p.poff[s] + r] = acc[a][b];
+            }
+        }
+}
+
+__global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
 {
     const uint32_t c = blockIdx.x;
-    const uint32_t P = p.P[c], D = p.D[c], K = p.K[c];
-    const double *X = p.X + p.xoff[c];
+    if (!p.active[c]) return;
+    const uint32_t P = p.P[c], K = p.K[c];
+    const double *xn = p.xnorm + p.poff[c];
+    const double *Gs = p.G + p.goff[c];
     double *cd = p.cdist + p.poff[c];
     uint32_t *cent = p.cent + p.koff[c];
-    __shared__ double s_val[KWG];
-    __shared__ uint32_t s_idx[KWG];
-    for (uint32_t v = threadIdx.x; v < P; v += KWG) cd[v] = 0.0;
-    if (threadIdx.x == 0) cent[0] = 0;
+    __shared__ double s_val[NWAVE];
+    __shared__ uint32_t s_idx[NWAVE];
+    __shared__ uint32_t s_last;
+    for (uint32_t v = threadIdx.x; v < P; v += SWG) cd[v] = 0.0;
+    if (threadIdx.x == 0) {
+        cent[0] = 0;
+        s_last = 0;
+    }
     __syncthreads();
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     for (uint32_t i = 1; i < K; ++i) {
-        const uint32_t last = cent[i - 1];
+        const uint32_t last = s_last;
         const double pr = 1.0 / i;
         const double q = 1.0 - pr;
+        const double *row = Gs + (uint64_t)last * P;
+        const double xl = xn[last];
         double best = -1.0;
         uint32_t bidx = 0xFFFFFFFFu;
-        for (uint32_t v = threadIdx.x; v < P; v += KWG) {
-            const double d = sq_rows(X + (uint64_t)last * D, X + (uint64_t)v * D, D);
+        for (uint32_t v = threadIdx.x; v < P; v += SWG) {
+            const double d = (xl + xn[v]) - 2.0 * row[v];
             const double nv = cd[v] * q + d * pr;
             cd[v] = nv;
             if (nv > best) {            /* v ascending within a thread: first wins */
@@ -67,212 +168,198 @@ __global__ __launch_bounds__(KWG) void k_km_init(KmParams p)
                 bidx = v;
             }
         }
-        s_val[threadIdx.x] = best;
-        s_idx[threadIdx.x] = bidx;
-        __syncthreads();
-        for (int s = KWG / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
-                const double ov = s_val[threadIdx.x + s];
-                const uint32_t oi = s_idx[threadIdx.x + s];
-                if (ov > s_val[threadIdx.x] || (ov == s_val[threadIdx.x] && oi < s_idx[threadIdx.x])) {
-                    s_val[threadIdx.x] = ov;
-                    s_idx[threadIdx.x] = oi;
-                }
-            }
-            __syncthreads();
-        }
         /* largest value, smallest index among equals: the reference's scan
          * from point 0 with a strict '>' */
-        if (threadIdx.x == 0) cent[i] = s_idx[0];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(best, o, 64);
+            const uint32_t oi = __shfl_xor(bidx, o, 64);
+            if (ov > best || (ov == best && oi < bidx)) {
+                best = ov;
+                bidx = oi;
+            }
+        }
+        if (lane == 0) {
+            s_val[wave] = best;
+            s_idx[wave] = bidx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = s_val[0];
+            uint32_t bi = s_idx[0];
+            for (int w = 1; w < NWAVE; ++w)
+                if (s_val[w] > b || (s_val[w] == b && s_idx[w] < bi)) {
+                    b = s_val[w];
+                    bi = s_idx[w];
+                }
+            cent[i] = bi;
+            s_last = bi;
+        }
         __syncthreads();
     }
 }
 
-/* grid: one workgroup per 64-point tile; tile_cl / tile_p0 map tiles */
-__global__ __launch_bounds__(KWG) void k_km_assign(KmParams p, const uint32_t *tile_cl, const uint32_t *tile_p0)
+/* grid: one workgroup per 256-point tile; tile_cl / tile_p0 map tiles */
+__global__ __launch_bounds__(AWG) void k_km_assign(KmParams p, const uint32_t *tile_cl, const uint32_t *tile_p0)
 {
     const uint32_t c = tile_cl[blockIdx.x];
     if (!p.active[c]) return;
-    const uint32_t p0 = tile_p0[blockIdx.x];
-    const uint32_t P = p.P[c], D = p.D[c], K = p.K[c];
-    const double *X = p.X + p.xoff[c];
+    const uint32_t P = p.P[c], K = p.K[c];
     const double *xn = p.xnorm + p.poff[c];
+    const double *Gs = p.G + p.goff[c];
     const uint32_t *cent = p.cent + p.koff[c];
-    __shared__ double sx[TD][TP + 1];
-    __shared__ double sc[TD][TC + 1];
-    __shared__ double s_best[16][TP];
-    __shared__ uint32_t s_bidx[16][TP];
-    const int tx = threadIdx.x % 16;   /* centroid sub-block */
-    const int ty = threadIdx.x / 16;   /* point sub-block */
-    double best[4];
-    uint32_t bidx[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        best[a] = 0;
-        bidx[a] = 0xFFFFFFFFu;
-    }
-    for (uint32_t c0 = 0; c0 < K; c0 += TC) {
-        double acc[4][4] = {};
-        for (uint32_t d0 = 0; d0 < D; d0 += TD) {
-            for (int e = threadIdx.x; e < TD * TP; e += KWG) {
-                const int dd = e / TP, pp = e % TP;
-                const uint32_t v = p0 + pp, f = d0 + dd;
-                sx[dd][pp] = (v < P && f < D) ? X[(uint64_t)v * D + f] : 0.0;
-            }
-            for (int e = threadIdx.x; e < TD * TC; e += KWG) {
-                const int dd = e / TC, cc = e % TC;
-                const uint32_t k = c0 + cc, f = d0 + dd;
-                sc[dd][cc] = (k < K && f < D) ? X[(uint64_t)cent[k] * D + f] : 0.0;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int dd = 0; dd < TD; ++dd) {
-                double xv[4], cv[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    xv[a] = sx[dd][ty * 4 + a];
-                    cv[a] = sc[dd][tx * 4 + a];
-                }
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fma(xv[a], cv[b], acc[a][b]);
-            }
-            __syncthreads();
+    __shared__ uint32_t s_u[AWG];
+    __shared__ double s_n[AWG];
+    const uint32_t v = tile_p0[blockIdx.x] + threadIdx.x;
+    const bool valid = v < P;
+    double best = 0;
+    uint32_t bk = 0xFFFFFFFFu;
+    for (uint32_t k0 = 0; k0 < K; k0 += AWG) {
+        __syncthreads();
+        if (k0 + threadIdx.x < K) {
+            const uint32_t u = cent[k0 + threadIdx.x];
+            s_u[threadIdx.x] = u;
+            s_n[threadIdx.x] = xn[u];
         }
-        /* score = ||c||^2 - 2 c.x (exact); centroids ascending within a thread */
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t k = c0 + tx * 4 + b;
-            if (k >= K) continue;
-            const double cn = xn[cent[k]];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double s = cn - 2.0 * acc[a][b];
-                if (bidx[a] == 0xFFFFFFFFu || s < best[a]) {
-                    best[a] = s;
-                    bidx[a] = k;
-                }
+        __syncthreads();
+        const uint32_t kn = K - k0 < (uint32_t)AWG ? K - k0 : (uint32_t)AWG;
+        if (!valid) continue;
+        /* score ||c||^2 - 2 c.x ranks the centroids as ||x - c||^2 does (exact);
+         * centroids ascending: the first of equals is kept */
+#pragma unroll 8
+        for (uint32_t j = 0; j < kn; ++j) {
+            const double sc = s_n[j] - 2.0 * Gs[(uint64_t)s_u[j] * P + v];
+            if (bk == 0xFFFFFFFFu || sc < best) {
+                best = sc;
+                bk = k0 + j;
             }
         }
     }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        s_best[tx][ty * 4 + a] = best[a];
-        s_bidx[tx][ty * 4 + a] = bidx[a];
-    }
-    __syncthreads();
-    if (threadIdx.x < TP) {
-        const uint32_t v = p0 + threadIdx.x;
-        if (v < P) {
-            double b = 0;
-            uint32_t bi = 0xFFFFFFFFu;
-            for (int t = 0; t < 16; ++t) {
-                const uint32_t i = s_bidx[t][threadIdx.x];
-                if (i == 0xFFFFFFFFu) continue;
-                const double s = s_best[t][threadIdx.x];
-                if (bi == 0xFFFFFFFFu || s < b || (s == b && i < bi)) {
-                    b = s;
-                    bi = i;
-                }
-            }
-            p.assign[p.poff[c] + v] = bi;
-        }
-    }
+    if (valid) p.assign[p.poff[c] + v] = bk;
 }
 
-__global__ __launch_bounds__(KWG) void k_km_update(KmParams p, uint32_t iteration, uint32_t max_iteration)
+__global__ __launch_bounds__(SWG) void k_km_update(KmParams p, uint32_t iteration, uint32_t max_iteration)
 {
     const uint32_t c = blockIdx.x;
     if (!p.active[c]) return;
     const uint32_t P = p.P[c], D = p.D[c], K = p.K[c];
     const double *X = p.X + p.xoff[c];
+    const double *xn = p.xnorm + p.poff[c];
+    const double *Gs = p.G + p.goff[c];
     const uint32_t *asg = p.assign + p.poff[c];
     uint32_t *cent = p.cent + p.koff[c];
     uint32_t *next = p.next + p.koff[c];
     uint32_t *cnt = p.cnt + p.koff[c];
     uint32_t *start = p.start + p.koff[c];
     uint32_t *members = p.members + p.poff[c];
+    double *md = p.mdist + p.poff[c];
     double *mean = p.mean + p.moff[c];
-    __shared__ double s_var[KWG];
+    __shared__ uint32_t s_cnt[KM_MAX_K];
+    __shared__ uint32_t s_start[KM_MAX_K];
+    __shared__ uint32_t s_wave[NWAVE];
+    __shared__ double s_var[NWAVE];
 
-    /* members of each centroid in point order (stable counting sort) */
-    if (threadIdx.x == 0) {
-        for (uint32_t k = 0; k < K; ++k) cnt[k] = 0;
-        for (uint32_t v = 0; v < P; ++v) cnt[asg[v]]++;
-        uint32_t run = 0;
-        for (uint32_t k = 0; k < K; ++k) {
+    /* members grouped by centroid (the order inside a group is irrelevant:
+     * sums are exact and the nearest member is chosen by (distance, index)) */
+    for (uint32_t k = threadIdx.x; k < K; k += SWG) s_cnt[k] = 0;
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < P; v += SWG) atomicAdd(&s_cnt[asg[v]], 1u);
+    __syncthreads();
+    {
+        const uint32_t per = (K + SWG - 1) / SWG, k0 = threadIdx.x * per;
+        uint32_t part = 0;
+        for (uint32_t k = k0; k < k0 + per && k < K; ++k) part += s_cnt[k];
+        uint32_t total;
+        uint32_t run = block_scan(part, s_wave, &total);
+        for (uint32_t k = k0; k < k0 + per && k < K; ++k) {
+            s_start[k] = run;
             start[k] = run;
-            run += cnt[k];
+            cnt[k] = s_cnt[k];
+            run += s_cnt[k];
         }
-        for (uint32_t k = 0; k < K; ++k) cnt[k] = 0;
-        for (uint32_t v = 0; v < P; ++v) {
-            const uint32_t k = asg[v];
-            members[start[k] + cnt[k]++] = v;
-        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += SWG) s_cnt[k] = 0;
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < P; v += SWG) {
+        const uint32_t k = asg[v];
+        members[s_start[k] + atomicAdd(&s_cnt[k], 1u)] = v;
     }
     __syncthreads();
     /* means: exact integer sums divided once by the member count */
-    for (uint64_t e = threadIdx.x; e < (uint64_t)K * D; e += KWG) {
+    for (uint64_t e = threadIdx.x; e < (uint64_t)K * D; e += SWG) {
         const uint32_t k = (uint32_t)(e / D), f = (uint32_t)(e % D);
-        const uint32_t n = cnt[k];
+        const uint32_t n = s_cnt[k];
         if (!n) continue;
         double s = 0;
-        for (uint32_t i = 0; i < n; ++i) s += X[(uint64_t)members[start[k] + i] * D + f];
+        for (uint32_t i = 0; i < n; ++i) s += X[(uint64_t)members[s_start[k] + i] * D + f];
         mean[e] = s / (double)n;
     }
     __syncthreads();
-    /* nearest member to each mean, reference order and first-wins */
-    for (uint32_t k = threadIdx.x; k < K; k += KWG) {
-        const uint32_t n = cnt[k];
-        if (!n) {
-            next[k] = cent[k];
-            continue;
-        }
-        const double *m = mean + (uint64_t)k * D;
-        uint32_t bv = members[start[k]];
-        double bd = sq_rows(m, X + (uint64_t)bv * D, D);
-        for (uint32_t i = 1; i < n; ++i) {
-            const uint32_t v = members[start[k] + i];
-            const double d = sq_rows(m, X + (uint64_t)v * D, D);
-            if (d < bd) {
-                bv = v;
-                bd = d;
+    /* distance of every member to its centroid's mean, reference order */
+    for (uint32_t i = threadIdx.x; i < P; i += SWG) {
+        const uint32_t v = members[i];
+        md[i] = sq_rows(mean + (uint64_t)asg[v] * D, X + (uint64_t)v * D, D);
+    }
+    __syncthreads();
+    /* nearest member to each mean: smallest distance, then smallest index
+     * (the reference scans members in point order with a strict '<') */
+    double var = 0;
+    for (uint32_t k = threadIdx.x; k < K; k += SWG) {
+        const uint32_t n = s_cnt[k], b0 = s_start[k];
+        uint32_t bv = cent[k];
+        if (n) {
+            bv = members[b0];
+            double bd = md[b0];
+            for (uint32_t i = 1; i < n; ++i) {
+                const uint32_t v = members[b0 + i];
+                const double d = md[b0 + i];
+                if (d < bd || (d == bd && v < bv)) {
+                    bv = v;
+                    bd = d;
+                }
             }
         }
         next[k] = bv;
-    }
-    __syncthreads();
-    double var = 0;
-    for (uint32_t k = threadIdx.x; k < K; k += KWG) {
-        const double d = sq_rows(X + (uint64_t)cent[k] * D, X + (uint64_t)next[k] * D, D);
+        const double d = gram_dist(Gs, xn, P, cent[k], bv);
         if (d > var) var = d;
     }
-    s_var[threadIdx.x] = var;
-    __syncthreads();
-    for (int s = KWG / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s && s_var[threadIdx.x + s] > s_var[threadIdx.x]) s_var[threadIdx.x] = s_var[threadIdx.x + s];
-        __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(var, o, 64);
+        if (ov > var) var = ov;
     }
-    const bool go_on = s_var[0] > p.min_variance && iteration <= max_iteration;
+    if (threadIdx.x % 64 == 0) s_var[threadIdx.x / 64] = var;
+    __syncthreads();
+    double vmax = s_var[0];
+    for (int w = 1; w < NWAVE; ++w) vmax = s_var[w] > vmax ? s_var[w] : vmax;
+    const bool go_on = vmax > p.min_variance && iteration <= max_iteration;
     if (!go_on) {
         /* final: the centroids of this assignment and their members' weights */
         const int32_t *w = p.weights + p.poff[c];
         int32_t *cw = p.cweights + p.koff[c];
-        for (uint32_t k = threadIdx.x; k < K; k += KWG) {
+        for (uint32_t k = threadIdx.x; k < K; k += SWG) {
             int32_t s = 0;
-            for (uint32_t i = 0; i < cnt[k]; ++i) s += w[members[start[k] + i]];
+            for (uint32_t i = 0; i < s_cnt[k]; ++i) s += w[members[s_start[k] + i]];
             cw[k] = s;
         }
         if (threadIdx.x == 0) p.active[c] = 0;
     } else {
-        for (uint32_t k = threadIdx.x; k < K; k += KWG) cent[k] = next[k];
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < K; k += SWG) cent[k] = next[k];
     }
+}
+
+hipError_t km_launch_gram(const KmParams &p, double *G, uint32_t ntiles, const uint32_t *tiles, hipStream_t s)
+{
+    if (!ntiles) return hipSuccess;
+    hipLaunchKernelGGL(k_km_gram, dim3(ntiles), dim3(GWG), 0, s, p, G, tiles);
+    return hipGetLastError();
 }
 
 hipError_t km_launch_init(const KmParams &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_km_init, dim3(p.ncl), dim3(KWG), 0, s, p);
+    hipLaunchKernelGGL(k_km_init, dim3(p.ncl), dim3(SWG), 0, s, p);
     return hipGetLastError();
 }
 
@@ -280,10 +367,10 @@ hipError_t km_launch_iteration(const KmParams &p, uint32_t ntiles, const uint32_
                                const uint32_t *tile_p0, uint32_t iteration, uint32_t max_iteration,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL(k_km_assign, dim3(ntiles), dim3(KWG), 0, s, p, tile_cl, tile_p0);
+    hipLaunchKernelGGL(k_km_assign, dim3(ntiles), dim3(AWG), 0, s, p, tile_cl, tile_p0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_km_update, dim3(p.ncl), dim3(KWG), 0, s, p, iteration, max_iteration);
+    hipLaunchKernelGGL(k_km_update, dim3(p.ncl), dim3(SWG), 0, s, p, iteration, max_iteration);
     return hipGetLastError();
 }
 

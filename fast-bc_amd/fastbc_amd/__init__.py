@@ -96,8 +96,8 @@ def lib():
         L.fbc_kmeans.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, i32p, i32p, ctypes.c_double,
                                  ctypes.c_int, i32p, i32p]
         L.fbc_clustered_bc.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, _u64p, ctypes.c_int,
-                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _f64p, i32p, _f64p,
-                                       _u32p, _u32p, _u32p, _f64p]
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p, i32p,
+                                       _f64p, _u32p, _u32p, _u32p, _f64p]
         L.fbc_exact_bc_arcs.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, ctypes.c_int, _f64p]
         _lib = L
     return _lib
@@ -365,8 +365,15 @@ PHASES = ("louvain_s", "clusters_s", "pivot_selection_s", "pivot_phase_s", "tota
           "cluster_te", "pivot_te", "pivot_kernel_ms")
 
 
-def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, device=0):
-    """End-to-end ClusteredBrandeBC::computeBC on the GPU; returns a dict."""
+FORM_BATCHED, FORM_PLAIN, FORM_SELECT_ONLY = 0, 1, 2
+
+
+def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, device=0,
+                 form=FORM_BATCHED):
+    """End-to-end ClusteredBrandeBC::computeBC on the GPU; returns a dict.
+    form: FORM_BATCHED (all pivots in one launch), FORM_PLAIN (one whole-GPU
+    launch per pivot, FP64) or FORM_SELECT_ONLY (stop after pivot selection;
+    bc is then the intra-cluster BC)."""
     s, d, ww = _arcs_i32(src, dst, w)
     n = int(s.max()) + 1
     bc = np.zeros(n)
@@ -377,7 +384,8 @@ def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, d
     pc = np.zeros(n, np.uint32)
     ph = np.zeros(8)
     k = _host_check(lib().fbc_clustered_bc(device, len(s), s, d, ww, np.ascontiguousarray(seeds, np.uint64),
-                                           len(seeds), precision, kfrac, prec, bc, comm, intra, piv, pw, pc, ph),
+                                           len(seeds), precision, kfrac, prec, form, bc, comm, intra, piv, pw,
+                                           pc, ph),
                     "fbc_clustered_bc")
     return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=pw[:k], pivot_cluster=pc[:k],
                 phases=dict(zip(PHASES, ph.tolist())))

@@ -10,8 +10,10 @@
 #define FBC_HOST_H
 
 #include "fbc_gpu.h"
+#include "../../include/fbc_host.h"
 
 #include <cstdint>
+#include <functional>
 #include <istream>
 #include <mutex>
 #include <string>
@@ -103,6 +105,7 @@ private:
 
 struct PhaseTimes {
     double louvain = 0, cluster = 0, pivots = 0, pivot_phase = 0, total = 0, kmeans = 0;
+    double kmeans_prep = 0, kmeans_gpu_ms = 0;
     size_t kmeans_gpu_sets = 0, kmeans_host_sets = 0;
     double cluster_te = 0, pivot_te = 0, cluster_kernel_ms = 0, pivot_kernel_ms = 0;
     size_t clusters = 0, bordered = 0, pivot_count = 0;
@@ -113,6 +116,10 @@ struct ClusteredOptions {
     double precision = 0.01;
     double kfrac = 0;            /* <= 0: plain VertexInfoPivotSelector */
     int prec = FBC_PREC_F64;     /* dependency arithmetic of the pivot phase */
+    int form = FBC_FORM_BATCHED; /* FBC_FORM_*: batched, plain, or stop after selection */
+    /* progress messages as the phases go (the reference's log lines at
+     * level 0 = info; per-chunk lines at level 1 = debug); may be empty */
+    std::function<void(int, const std::string &)> progress;
 };
 
 struct ClusteredRun {

@@ -13,6 +13,8 @@
  *   options of this build:
  *         --device             HIP device (default 0)
  *         --precision          f64 | f32c: dependency arithmetic (default f64)
+ *         --form               batched | plain: all pivots in one launch (default),
+ *                              or one whole-GPU launch per pivot (FP64)
  *         --full-output        also write full-precision, unclamped BC values
  *
  * Option grammar (popl 1.2.0 as vendored by the reference, popl.hpp:919-1024):
@@ -184,9 +186,10 @@ int main(int argc, char **argv)
         {'d', "debug", "Logger level (trace|debug|info|warning|error|critical|off)", true},
         {0, "device", "HIP device index (this build)", true},
         {0, "precision", "Dependency arithmetic: f64 | f32c (this build)", true},
+        {0, "form", "Pivot phase: batched | plain (this build)", true},
         {0, "full-output", "Also write unclamped %.17g values to this file (this build)", true},
     };
-    std::string seedstr, outpath = "bc.txt", level = "info", fullout, precstr = "f64";
+    std::string seedstr, outpath = "bc.txt", level = "info", fullout, precstr = "f64", formstr = "batched";
     int instances = 4, threads = 0, device = 0;
     double precision = 0.01, kfrac = 0;
     bool exact = false;
@@ -201,10 +204,15 @@ int main(int argc, char **argv)
         if (Opt *o = op.find('d'); o->set) level = o->value;
         if (Opt *o = op.find(std::string("device")); o->set) device = parse_value<int>(*o, true);
         if (Opt *o = op.find(std::string("precision")); o->set) precstr = o->value;
+        if (Opt *o = op.find(std::string("form")); o->set) formstr = o->value;
         if (Opt *o = op.find(std::string("full-output")); o->set) fullout = o->value;
         exact = op.find(std::string("exact"))->set;
         if (precstr != "f64" && precstr != "f32c")
             throw InvalidOption("invalid argument for --precision: '" + precstr + "'");
+        if (formstr != "batched" && formstr != "plain")
+            throw InvalidOption("invalid argument for --form: '" + formstr + "'");
+        if (formstr == "plain" && precstr != "f64")
+            throw InvalidOption("--form plain computes in f64");
     } catch (const InvalidOption &e) {
         std::cout << e.what() << "\n\n" << op.help();
         return -1;
@@ -291,7 +299,8 @@ int main(int argc, char **argv)
         opt.precision = precision;
         opt.kfrac = kfrac_set ? kfrac : 0.0;
         opt.prec = prec;
-        log(INFO, "Computing clusters with Louvain algorithm...");
+        opt.form = formstr == "plain" ? FBC_FORM_PLAIN : FBC_FORM_BATCHED;
+        opt.progress = [](int level, const std::string &m) { log(level == 0 ? INFO : DEBUG, m); };
         run = fbchost::clustered_bc(dev, g, opt);
         bc = run.bc;
     }
@@ -299,9 +308,6 @@ int main(int argc, char **argv)
     const long long ms = std::chrono::duration_cast<std::chrono::milliseconds>(total).count();
     const long long us = std::chrono::duration_cast<std::chrono::microseconds>(total).count() - ms * 1000;
     if (!exact) {
-        log(INFO, "Graph partitioned in " + std::to_string(run.t.clusters) + " clusters");
-        log(INFO, "Evaluating intra cluster BC...");
-        log(INFO, "Computing global BC from " + std::to_string(run.t.pivot_count) + " pivots...");
         char buf[256];
         std::snprintf(buf, sizeof buf,
                       "Phases: louvain %.3fs, clusters %.3fs, pivot selection %.3fs, pivot phase %.3fs",

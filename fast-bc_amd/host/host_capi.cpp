@@ -2,6 +2,8 @@
 #include "fbc_host.h"
 #include "../../include/fbc_host.h"
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
@@ -79,11 +81,12 @@ int fbc_kmeans(int k, int P, int nb, const double *vi_len, const int32_t *vi_cnt
 
 int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *dst, const double *w,
                      const uint64_t *seeds, int nseeds, double precision, double kfrac, int prec,
-                     double *bc, int32_t *comm_of, double *intra_bc, uint32_t *pivots, uint32_t *weights,
-                     uint32_t *pivot_cluster, double *phase_seconds)
+                     int form, double *bc, int32_t *comm_of, double *intra_bc, uint32_t *pivots,
+                     uint32_t *weights, uint32_t *pivot_cluster, double *phase_seconds)
 {
     if ((m && (!src || !dst || !w)) || !seeds || nseeds < 1 || !bc) return FBC_E_ARG;
     if (prec != FBC_PREC_F64 && prec != FBC_PREC_F32C) return FBC_E_ARG;
+    if (form != FBC_FORM_BATCHED && form != FBC_FORM_PLAIN && form != FBC_FORM_SELECT_ONLY) return FBC_E_ARG;
     return guarded([&] {
         Graph g = from_arcs(m, src, dst, w);
         Device dev(device);
@@ -92,6 +95,9 @@ int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *
         opt.precision = precision;
         opt.kfrac = kfrac;
         opt.prec = prec;
+        opt.form = form;
+        if (std::getenv("FBC_PROGRESS"))   /* progress lines on stderr */
+            opt.progress = [](int, const std::string &m) { std::fprintf(stderr, "[fbc] %s\n", m.c_str()); };
         ClusteredRun r = clustered_bc(dev, g, opt);
         std::memcpy(bc, r.bc.data(), r.bc.size() * sizeof(double));
         if (comm_of)

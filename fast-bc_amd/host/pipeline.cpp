@@ -17,6 +17,7 @@
 #include "fbc_host.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <numeric>
@@ -50,30 +51,43 @@ void kmeans_stage(Device &dev, uint32_t ncl, const std::vector<uint32_t> &bptr, 
         double mx = 0;
         for (double x : plen[c]) mx = std::max(mx, std::fabs(x));
         for (int x : pcnt[c]) mx = std::max(mx, std::fabs((double)x));
-        (2.0 * 2 * nb * mx * mx < 9007199254740992.0 ? gsets : hsets).push_back(c);
+        const int kc = std::max((int)(prow[c].size() * kfrac), 1);
+        const bool exact_on_gpu = 2.0 * 2 * nb * mx * mx < 9007199254740992.0 && kc <= FBC_KMEANS_MAX_K;
+        (exact_on_gpu ? gsets : hsets).push_back(c);
     }
     std::vector<std::vector<int>> cent(ncl), cw(ncl);
     if (!gsets.empty()) {
         std::vector<uint32_t> np, nf, kk;
-        std::vector<double> feat;
-        std::vector<int32_t> w;
-        for (uint32_t c : gsets) {
+        std::vector<size_t> foff(gsets.size() + 1, 0), woff(gsets.size() + 1, 0);
+        for (size_t i = 0; i < gsets.size(); ++i) {
+            const uint32_t c = gsets[i];
             const uint32_t P = (uint32_t)prow[c].size(), nb = bptr[c + 1] - bptr[c];
             np.push_back(P);
             nf.push_back(2 * nb);
             kk.push_back((uint32_t)std::max((int)(P * kfrac), 1));
-            for (uint32_t p = 0; p < P; ++p)
-                for (uint32_t j = 0; j < nb; ++j) {
-                    feat.push_back(plen[c][(size_t)p * nb + j]);
-                    feat.push_back((double)pcnt[c][(size_t)p * nb + j]);
-                }
-            w.insert(w.end(), pw[c].begin(), pw[c].end());
+            foff[i + 1] = foff[i] + (size_t)P * 2 * nb;
+            woff[i + 1] = woff[i] + P;
+        }
+        /* feature rows: length then count per border */
+        std::vector<double> feat(foff.back());
+        std::vector<int32_t> w(woff.back());
+#pragma omp parallel for schedule(dynamic, 1)
+        for (size_t i = 0; i < gsets.size(); ++i) {
+            const uint32_t c = gsets[i], P = np[i], nb = nf[i] / 2;
+            double *f = feat.data() + foff[i];
+            for (size_t e = 0; e < (size_t)P * nb; ++e) {
+                f[2 * e] = plen[c][e];
+                f[2 * e + 1] = (double)pcnt[c][e];
+            }
+            std::copy(pw[c].begin(), pw[c].end(), w.begin() + woff[i]);
         }
         const size_t ktot = std::accumulate(kk.begin(), kk.end(), (size_t)0);
         std::vector<uint32_t> oc(ktot);
         std::vector<int32_t> ow(ktot);
+        t.kmeans_prep = now() - t0;
         dev.check(fbc_kmeans_batch(dev.ctx(), (uint32_t)gsets.size(), np.data(), nf.data(), kk.data(), feat.data(),
                                    w.data(), 0.0, 100, oc.data(), ow.data()));
+        fbc_last_stats(dev.ctx(), &t.kmeans_gpu_ms, nullptr);
         size_t o = 0;
         for (size_t i = 0; i < gsets.size(); ++i) {
             cent[gsets[i]].assign(oc.begin() + o, oc.begin() + o + kk[i]);
@@ -180,10 +194,16 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     if (g.n == 0) return run;
 
     double t0 = now();
+    if (opt.progress) opt.progress(0, "Computing clusters with Louvain algorithm...");
     const std::vector<std::vector<int>> comms = louvain(g, opt.seeds, opt.precision);
     run.t.louvain = now() - t0;
     const uint32_t ncl = (uint32_t)comms.size();
     run.t.clusters = ncl;
+    auto report = [&](int level, const std::string &msg) {
+        if (opt.progress) opt.progress(level, msg);
+    };
+    report(0, "Graph partitioned in " + std::to_string(ncl) + " clusters");
+    report(0, "Evaluating intra cluster BC...");
 
     std::lock_guard<std::mutex> lk(dev.lock());
     fbc_graph *dg = dev.graph(g);
@@ -216,6 +236,7 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     dev.check(fbc_cluster_eval(dev.ctx(), cl, run.intra.data(), vl.data(), vc.data()));
     fbc_last_stats(dev.ctx(), &run.t.cluster_kernel_ms, &run.t.cluster_te);
     run.t.cluster = now() - t0;
+    report(1, "Intra-cluster BC evaluated");
 
     /* exact pivots per cluster (the selectors only read their own cluster);
      * the normalized rows of the exact pivots are kept for k-means */
@@ -254,7 +275,22 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
                 }
         }
     }
-    if (opt.kfrac > 0) kmeans_stage(dev, ncl, bptr, opt.kfrac, prow, pw, plen, pcnt, run.t);
+    {
+        size_t exact = 0;
+        for (uint32_t c = 0; c < ncl; ++c) exact += prow[c].size();
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "exact pivot classes: %zu (%.3f s)", exact, now() - t0);
+        report(1, buf);
+    }
+    if (opt.kfrac > 0) {
+        kmeans_stage(dev, ncl, bptr, opt.kfrac, prow, pw, plen, pcnt, run.t);
+        char buf[160];
+        std::snprintf(buf, sizeof buf,
+                      "k-means: %zu sets on the GPU, %zu on the host (%.3f s; feature rows %.3f s, GPU kernels %.3f s)",
+                      run.t.kmeans_gpu_sets, run.t.kmeans_host_sets, run.t.kmeans, run.t.kmeans_prep,
+                      run.t.kmeans_gpu_ms * 1e-3);
+        report(1, buf);
+    }
     for (uint32_t c = 0; c < ncl; ++c)
         for (size_t p = 0; p < prow[c].size(); ++p) piv[c].push_back((int)verts[cptr[c] + prow[c][p]]);
     for (uint32_t c = 0; c < ncl; ++c) {
@@ -268,11 +304,32 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     run.t.pivot_count = run.pivots.size();
     run.t.pivots = now() - t0;
 
+    report(0, "Computing global BC from " + std::to_string(run.pivots.size()) + " pivots...");
+
+    /* the pivot terms are additive, so the pivots go in chunks (bounded
+     * launches, progress between them) */
     t0 = now();
     run.bc = run.intra;
-    dev.check(fbc_pivot_phase(dev.ctx(), cl, run.pivots.data(), run.weights.data(), run.pivot_cluster.data(),
-                              (uint32_t)run.pivots.size(), run.intra.data(), opt.prec, run.bc.data()));
-    fbc_last_stats(dev.ctx(), &run.t.pivot_kernel_ms, &run.t.pivot_te);
+    if (opt.form != FBC_FORM_SELECT_ONLY) {
+        const size_t np = run.pivots.size();
+        const size_t chunk = opt.form == FBC_FORM_PLAIN ? 256 : (size_t)1 << 16;
+        for (size_t p0 = 0; p0 < np; p0 += chunk) {
+            const uint32_t k = (uint32_t)std::min(chunk, np - p0);
+            if (opt.form == FBC_FORM_PLAIN)
+                dev.check(fbc_pivot_phase_plain(dev.ctx(), cl, run.pivots.data() + p0, run.weights.data() + p0,
+                                                run.pivot_cluster.data() + p0, k, run.intra.data(),
+                                                run.bc.data()));
+            else
+                dev.check(fbc_pivot_phase(dev.ctx(), cl, run.pivots.data() + p0, run.weights.data() + p0,
+                                          run.pivot_cluster.data() + p0, k, run.intra.data(), opt.prec,
+                                          run.bc.data()));
+            double ms = 0, te = 0;
+            fbc_last_stats(dev.ctx(), &ms, &te);
+            run.t.pivot_kernel_ms += ms;
+            run.t.pivot_te += te;
+            if (np > chunk) report(1, "pivots done: " + std::to_string(p0 + k) + " / " + std::to_string(np));
+        }
+    }
     run.t.pivot_phase = now() - t0;
     run.t.total = now() - t_start;
     return run;

@@ -38,17 +38,25 @@ int fbc_kmeans(int k, int P, int nb, const double *vi_len, const int32_t *vi_cnt
                const int32_t *weights, double min_variance, int max_iteration,
                int32_t *centroids, int32_t *cweights);
 
+/* How fbc_clustered_bc runs its pivot phase. */
+enum {
+    FBC_FORM_BATCHED = 0,      /* every pivot in one persistent launch (fbc_pivot_phase) */
+    FBC_FORM_PLAIN = 1,        /* one whole-GPU launch per pivot, FP64 (fbc_pivot_phase_plain) */
+    FBC_FORM_SELECT_ONLY = 2   /* stop after pivot selection: bc = intra-cluster BC */
+};
+
 /* ClusteredBrandeBC::computeBC (brandes/ClusteredBrandesBC.h:63-169) as main.cpp
  * composes it (main.cpp:223-260): kfrac <= 0 selects the exact pivot selector.
  * bc[n] (pre-clamp), optional comm_of[n], intra_bc[n], pivots / weights /
  * pivot_cluster (capacity n), phase_seconds[8] = {louvain, clusters, pivot
  * selection, pivot phase, total, cluster traversed edges, pivot traversed
- * edges, pivot-phase kernel ms}.  Returns the pivot count. */
+ * edges, pivot-phase kernel ms}.  'form' is one of FBC_FORM_*; prec applies
+ * to the batched form.  Returns the pivot count. */
 int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *dst,
                      const double *w, const uint64_t *seeds, int nseeds, double precision,
-                     double kfrac, int prec, double *bc, int32_t *comm_of, double *intra_bc,
-                     uint32_t *pivots, uint32_t *weights, uint32_t *pivot_cluster,
-                     double *phase_seconds);
+                     double kfrac, int prec, int form, double *bc, int32_t *comm_of,
+                     double *intra_bc, uint32_t *pivots, uint32_t *weights,
+                     uint32_t *pivot_cluster, double *phase_seconds);
 
 /* ExactBrandesBC::computeBC (brandes/ExactBrandesBC.h:46-95) from arcs. */
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,

@@ -52,6 +52,10 @@ def test_cfg0_clustered_end_to_end(gpu_ctx, kfrac):
     check_bc(out["bc"], ref["bc"], term_magnitude(g, ref))
     f32 = F.clustered_bc(s, d, w, [1, 2, 3, 4], 0.01, kfrac=kfrac, prec=F.PREC_F32C)
     check_bc(f32["bc"], ref["bc"], term_magnitude(g, ref), rtol=1e-6)
+    plain = F.clustered_bc(s, d, w, [1, 2, 3, 4], 0.01, kfrac=kfrac, form=F.FORM_PLAIN)
+    check_bc(plain["bc"], ref["bc"], term_magnitude(g, ref))
+    sel = F.clustered_bc(s, d, w, [1, 2, 3, 4], 0.01, kfrac=kfrac, form=F.FORM_SELECT_ONLY)
+    assert np.array_equal(sel["pivots"], out["pivots"]) and np.array_equal(sel["bc"], ref["intra"])
 
 
 def test_cfg1_clustered_pivots_and_bc(gpu_ctx):
