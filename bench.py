@@ -1,20 +1,33 @@
-"""Benchmark: clustered-betweenness pivot phase on the 1,048,576-vertex road grid.
+"""Benchmark: clustered betweenness centrality on the 1,048,576-vertex road grid.
 
 Workload: the 1024x1024 road-like grid (seed 3: 10% of links deleted under a
-spanning-tree guard, 2% diagonal arcs, weights 1..1000; 3,846,612 arcs).
-One step = one launch of the global pivot phase
-(ClusteredBrandesBC.h:141-166) over a batch of pivot sources: a full-graph
-SSSP with exact uint64 path counts and the dependency sums per source,
-accumulated into the FP64 betweenness vector in HBM.  The headline form is the
-batched persistent kernel with FP32 two-sum dependency sums; the FP64 form is
-timed beside it.
+spanning-tree guard, 2% diagonal arcs, weights 1..1000; 3,846,612 arcs), run
+as main.cpp composes the clustered computer with -s 1,2,3,4 -e 4 -p 0.01
+-k 0.5 (ClusteredBrandeBC::computeBC, ClusteredBrandesBC.h:63-169).
 
-Output: one JSON line (see the repository's measurement notes) with
-  value     = traversed edges per second (GTEPS) of the headline form,
-  roofline  = 48 algorithmic bytes per traversed edge over the kernel's
-              HIP-event time, against 8 TB/s,
-  cpu_baseline = the reference's DijkstraSSBrandesBC (oracle/_ref build) on
-              the host cores over a bounded sample of the same sources.
+One step = one computeBC: Louvain on the host, every (cluster, source)
+Brandes pass in one persistent launch, the VertexInfo pivot classes and the
+k-means super-classes (GPU), then every pivot in one persistent launch -- the
+batched form, with FP32 two-sum dependency sums (the headline).  Timed like
+the reference's timer (main.cpp:269-277): computeBC only, graph building
+excluded; the library call is synchronous.
+
+Beside the headline, in the same JSON line:
+  fp64_form        the batched form with FP64 dependency sums (one run) and
+                   the largest relative BC difference between the two forms;
+  plain_fp64_form  the pivot phase as the reference runs it, one
+                   singleSourceBrandes per pivot -- here one cooperative
+                   whole-GPU launch per pivot, FP64 -- timed over a sample of
+                   the run's pivots and scaled to all of them; the other
+                   phases are the batched form's (a conservative baseline);
+  roofline         the pivot-phase launch (the dominant kernel): 48
+                   algorithmic bytes per traversed edge over its HIP-event
+                   time, against 8 TB/s; traffic from the rocprofv3 PMC
+                   summary in profiles/ when present;
+  cpu_baseline     the reference's DijkstraSSBrandesBC (oracle/_ref build) on
+                   the host cores over a bounded sample of the run's pivots.
+value = traversed edges of the step (cluster phase + pivot phase) per second
+of computeBC wall time.
 """
 import argparse
 import json
@@ -30,6 +43,8 @@ sys.path.insert(0, os.path.join(ROOT, "fast-bc_amd"))
 METRIC = "GTEPS + BC wall-time, 1M-vertex grid: batched FP32-comp vs plain FP64, 1 GPU"
 BYTES_PER_TE = 48.0
 HBM_PEAK_GBS = 8000.0
+SEEDS = [1, 2, 3, 4]
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_brandes_pivot_cfg2_traffic.json")
 
 
 def log(*a):
@@ -75,7 +90,7 @@ def cpu_baseline(n, arcs, sources, seconds_target):
                 k = int(min(len(sources), max(k, k * seconds_target / max(secs, 1e-3))))
                 k = max(cores, (k // cores) * cores)
         kind = "reference"
-        sample = (f"{k} pivot sources of the same batch through DijkstraSSBrandesBC::singleSourceBrandes "
+        sample = (f"{k} of the run's pivots through DijkstraSSBrandesBC::singleSourceBrandes "
                   f"(reference headers, g++ -O3, OpenMP reduction as ClusteredBrandesBC.h:141-155), "
                   f"{secs:.1f} s, graph load excluded")
     else:
@@ -89,20 +104,32 @@ def cpu_baseline(n, arcs, sources, seconds_target):
         secs = time.time() - t
         cores = 1
         kind = "port"
-        sample = f"{k} pivot sources through the oracle's single-source Brandes, one thread, {secs:.1f} s"
+        sample = f"{k} of the run's pivots through the oracle's single-source Brandes, one thread, {secs:.1f} s"
     return {"value": k * m / secs / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind, "sample": sample}
+
+
+def traffic_per_launch(npiv):
+    """HBM bytes of the pivot-phase launch from the committed PMC summary,
+    scaled from its pivot count to this run's."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            t = json.load(f)
+        return t["bytes_per_pivot"] * npiv, t
+    except (OSError, ValueError, KeyError):
+        return None, None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=2048, help="pivot sources per step")
-    ap.add_argument("--delta", type=int, default=0, help="near-far window (0 = automatic)")
-    ap.add_argument("--slots", type=int, default=0, help="sources in flight (0 = automatic)")
+    ap.add_argument("--kfrac", type=float, default=0.5)
+    ap.add_argument("--plain-sample", type=int, default=32, help="pivots timed in the plain form")
+    ap.add_argument("--plain-blocks", type=int, default=16, help="workgroups of the plain form's grid")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 and plain forms")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -116,8 +143,8 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the pivot queue would shard "
-              f"across GPUs as described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build measures a single "
+              f"MI355X; the pivot and (cluster, source) queues would be split across GPUs", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -126,40 +153,32 @@ def main():
     t0 = time.time()
     n, arcs = F.generate_config(2)
     m = len(arcs[0])
-    ctx = F.Context(0)
-    ctx.set_tuning(args.delta, args.slots)
-    g = F.Graph(ctx, n, *F.csr_from_arcs(n, *arcs))
-    # pivot batch: a seeded sample of vertices, every one its own weight-1 class
-    # of a single all-vertex cluster (the per-source work does not depend on it)
-    cl = g.clusters(comm_of=np.zeros(n, np.int64))
-    rng = np.random.default_rng(12345)
-    piv = rng.choice(n, args.batch, replace=False).astype(np.uint32)
-    wts = np.ones(args.batch, np.uint32)
-    pcl = np.zeros(args.batch, np.uint32)
-    intra = np.zeros(n)
-    log(f"setup {time.time() - t0:.1f}s: n={n} m={m} batch={args.batch}")
+    log(f"graph: n={n} m={m} ({time.time() - t0:.1f}s)")
 
-    def run_form(prec, steps, warmup):
-        bc = np.zeros(n)
-        for _ in range(warmup):
-            cl.pivot_phase(piv, wts, pcl, intra, bc, prec=prec)
-        kms, te = 0.0, 0.0
+    def step(prec, form=F.FORM_BATCHED):
         t = time.perf_counter()
-        for _ in range(steps):
-            bc = cl.pivot_phase(piv, wts, pcl, intra, bc, prec=prec)
-            ms, e = ctx.last_stats()
-            kms += ms
-            te += e
-        wall = time.perf_counter() - t
-        return wall, kms, te, ctx.last_counters()
+        r = F.clustered_bc(*arcs, SEEDS, 0.01, kfrac=args.kfrac, prec=prec, form=form)
+        r["call_s"] = time.perf_counter() - t
+        return r
 
-    wall, kms, te, counters = run_form(F.PREC_F32C, args.steps, args.warmup)
-    log(f"f32c: {args.steps} steps wall {wall:.3f}s kernel {kms:.1f}ms TE {te:.3e} counters {counters}")
-    wall64, kms64, te64, _ = run_form(F.PREC_F64, max(1, args.steps // 2), 0)
-    log(f"f64:  wall {wall64:.3f}s kernel {kms64:.1f}ms")
+    for i in range(args.warmup):
+        r = step(F.PREC_F32C)
+        log(f"warmup {i}: computeBC {r['phases']['total_s']:.2f}s")
+    runs = []
+    for i in range(args.steps):
+        r = step(F.PREC_F32C)
+        runs.append(r)
+        log(f"step {i}: computeBC {r['phases']['total_s']:.2f}s phases {r['phases']}")
+    last = runs[-1]
+    ph = last["phases"]
+    total_s = sum(r["phases"]["total_s"] for r in runs) / len(runs)
+    te = ph["cluster_te"] + ph["pivot_te"]
+    npiv = len(last["pivots"])
+    gteps = te / total_s / 1e9
+    piv_ms = ph["pivot_kernel_ms"]
+    achieved = ph["pivot_te"] * BYTES_PER_TE / (piv_ms * 1e-3) / 1e9
+    traffic, tsum = traffic_per_launch(npiv)
 
-    gteps = te / wall / 1e9
-    achieved = te / args.steps * BYTES_PER_TE / (kms / args.steps * 1e-3) / 1e9
     result = {
         "metric": METRIC,
         "value": round(gteps, 4),
@@ -167,30 +186,81 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(total_s * 1e3, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32x2 (two-sum) dependencies, uint32 paths, uint64 sigma, fp64 reduction",
         "data": "synthetic",
         "config": {
-            "workload": "1024x1024 road-like grid (seed 3), global pivot phase, one batch of pivot sources per step",
-            "vertices": n, "arcs": m, "batch_sources": args.batch,
-            "delta": args.delta or "auto", "slots": args.slots or "auto",
-            "te_per_step": te / args.steps,
+            "workload": "1024x1024 road-like grid (seed 3), clustered BC -s 1,2,3,4 -e 4 -p 0.01 -k 0.5, "
+                        "batched form, one computeBC per step",
+            "vertices": n, "arcs": m, "clusters": int(np.max(last["comm"])) + 1, "pivots": npiv,
+            "traversed_edges_per_step": te,
         },
+        "bc_wall_s": round(total_s, 3),
+        "phases_s": {k: round(v, 3) for k, v in ph.items() if k.endswith("_s")},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": round(traffic) if traffic else None,
+            "kernel": "k_brandes<MODE_PIVOT, F32C> (pivot phase, one launch)",
+            "kernel_ms": round(piv_ms, 2),
         },
-        "fp64_form": {"value": round(te64 / wall64 / 1e9, 4), "unit": "GTEPS",
-                      "kernel_ms_per_step": round(kms64 / max(1, args.steps // 2), 3)},
-        "kernel_counters": counters,
-        "pivot_phase_s_for_524288_pivots": round(524288 * m / (gteps * 1e9), 1),
     }
+    if tsum:
+        result["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+
+    if not args.no_compare:
+        r64 = step(F.PREC_F64)
+        p64 = r64["phases"]
+        bc32, bc64 = last["bc"], r64["bc"]
+        nz = bc64 != 0
+        rel = float(np.max(np.abs(bc32[nz] - bc64[nz]) / np.abs(bc64[nz]))) if nz.any() else 0.0
+        log(f"fp64 batched: computeBC {p64['total_s']:.2f}s; max rel diff fp32c vs fp64 {rel:.3e}")
+        result["fp64_form"] = {
+            "bc_wall_s": round(p64["total_s"], 3),
+            "value": round((p64["cluster_te"] + p64["pivot_te"]) / p64["total_s"] / 1e9, 4), "unit": "GTEPS",
+            "pivot_kernel_ms": round(p64["pivot_kernel_ms"], 2),
+            "fp32c_max_rel_diff": rel,
+            "same_pivots": bool(np.array_equal(r64["pivots"], last["pivots"])),
+        }
+        # the plain form over a sample of the same pivots
+        ctx = F.Context(0)
+        g = F.Graph(ctx, n, *F.csr_from_arcs(n, *arcs))
+        cl = g.clusters(comm_of=r64["comm"])
+        k = min(args.plain_sample, npiv)
+        sp, sw, sc = r64["pivots"][:k], r64["weights"][:k], r64["pivot_cluster"][:k]
+        zero = np.zeros(n)
+        ctx.set_tuning(0, args.plain_blocks)
+        t = time.perf_counter()
+        plain = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, plain=True)
+        plain_wall = time.perf_counter() - t
+        plain_ms, plain_te = ctx.last_stats()
+        ctx.set_tuning(0, 0)
+        batched = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, prec=F.PREC_F64)
+        scale = max(np.abs(batched).max(), 1e-300)
+        agree = float(np.max(np.abs(plain - batched)) / scale)
+        plain_pivot_s = plain_ms * 1e-3 / k * npiv
+        plain_total = p64["total_s"] - p64["pivot_phase_s"] + plain_pivot_s
+        log(f"plain: {k} pivots {plain_ms:.1f} ms ({plain_ms / k:.2f} ms/pivot); pivot phase for {npiv} "
+            f"pivots {plain_pivot_s:.1f}s; computeBC {plain_total:.1f}s; agreement {agree:.2e}")
+        result["plain_fp64_form"] = {
+            "bc_wall_s": round(plain_total, 2),
+            "value": round(te / plain_total / 1e9, 4), "unit": "GTEPS",
+            "pivot_phase_s": round(plain_pivot_s, 2),
+            "ms_per_pivot": round(plain_ms / k, 3),
+            "pivot_gteps": round(plain_te / (plain_ms * 1e-3) / 1e9, 4),
+            "sample": f"{k} of the run's {npiv} pivots, one cooperative launch of {args.plain_blocks} "
+                      f"workgroups each; pivot phase scaled by {npiv}/{k}",
+            "max_diff_vs_batched_fp64": agree,
+        }
+        result["speedup_batched_fp32c_vs_plain_fp64"] = round(plain_total / total_s, 2)
+        result["pivot_phase_speedup"] = round(plain_pivot_s / ph["pivot_phase_s"], 1)
+        del g, cl, ctx
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(n, arcs, piv, args.cpu_seconds)
+            result["cpu_baseline"] = cpu_baseline(n, arcs, last["pivots"], args.cpu_seconds)
         except Exception as e:  # the GPU figure stands without it
             log(f"cpu baseline failed: {e}")
             result["cpu_baseline"] = None

@@ -135,7 +135,9 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)(ctx->cu_count * 8);
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
-    const size_t budget = std::min<size_t>(ctx->total_mem / 3, (size_t)96 << 30);
+    /* whole-graph slots (tens of MB each) stop at ~2 per CU: beyond that the
+     * slots' working sets only contend for L2 and MALL */
+    const size_t budget = std::min<size_t>(ctx->total_mem / 3, (size_t)24 << 30);
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     const size_t need = (size_t)slots * sb;
     if (need > ctx->scratch_bytes) {
