@@ -225,7 +225,36 @@ struct Shared {
     uint32_t nheavy[2];   /* by round (level) parity */
     uint32_t heavy[HEAVY_CAP];
     unsigned long long te;
+    uint32_t sortbuf[SORT_CAP_MAX];
 };
+
+/* Sort q[0, n) ascending through LDS (bitonic; n <= SORT_CAP_MAX), so that a
+ * round's or a level's vertices are expanded in id order: neighbouring ids
+ * share cache lines of the per-vertex state and of the CSR. */
+__device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
+{
+    uint32_t p2 = 64;
+    while (p2 < n) p2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < p2; i += WG) buf[i] = i < n ? q[i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= p2; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p2; i += WG) {
+                const uint32_t o = i ^ j;
+                if (o > i) {
+                    const uint32_t a = buf[i], b = buf[o];
+                    if ((a > b) == ((i & k) == 0)) {
+                        buf[i] = b;
+                        buf[o] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < n; i += WG) q[i] = buf[i];
+    __syncthreads();
+}
+
 
 /* One workgroup per source (batched kernel). */
 struct TeamWG {
@@ -250,6 +279,9 @@ struct TeamGrid {
     __device__ void sync() const { cooperative_groups::this_grid().sync(); }
     __device__ bool aborted() const { return false; }
 };
+
+template <class T> struct IsWG { static constexpr bool value = false; };
+template <> struct IsWG<TeamWG> { static constexpr bool value = true; };
 
 /* Relax arc e of u (distance du); used for the arcs of heavy vertices, one
  * per thread. */
@@ -342,7 +374,7 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
  * counters and window minima alternate by window parity in the same way. */
 template <class T>
 __device__ __forceinline__ void phase_distances(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
-                                uint32_t delta, uint32_t *err, Counters &ct)
+                                uint32_t delta, uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
     const uint32_t r0 = t.rank(), R = t.size();
@@ -379,6 +411,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, Sha
                 ast32<SC>(&c.qcnt[(rnd + 2) % 3], 0u);
             }
             if (threadIdx.x == 0) sh.nheavy[(rnd + 1) & 1] = 0;
+            if (IsWG<T>::value && near > 64 && near <= sort_cap) sort_vertices(qa, near, sh.sortbuf);
             for (uint32_t i = r0; i < near; i += R) {
                 const uint32_t u = qa[i];
                 ++ct.c[STAT_VISITS];
@@ -489,7 +522,7 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
  * the high half keeps the distance. */
 template <class T>
 __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
-                                uint32_t *err, Counters &ct)
+                                uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
     const uint32_t r0 = t.rank(), R = t.size();
@@ -530,6 +563,7 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Sha
     for (uint32_t lvl = 0; lb < le; ++lvl) {
         uint32_t *nh_cnt = &sh.nheavy[lvl & 1];
         if (threadIdx.x == 0) sh.nheavy[(lvl + 1) & 1] = 0;
+        if (IsWG<T>::value && le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, sh.sortbuf);
         for (uint32_t i = lb + r0; i < le; i += R) {
             const uint32_t u = S.qa[i];
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
@@ -759,9 +793,9 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
         const uint32_t src = p.task_src[task];
 
         const uint64_t t0 = wall_clock64();
-        phase_distances(V, S, sh, t, src, p.delta, p.err, ct);
+        phase_distances(V, S, sh, t, src, p.delta, p.sort_cap, p.err, ct);
         const uint64_t t1 = wall_clock64();
-        const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.err, ct);
+        const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.sort_cap, p.err, ct);
         const uint64_t t2 = wall_clock64();
         if (tid == 0) {
             ct.c[STAT_T_DIST] += t1 - t0;
@@ -814,9 +848,9 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];
     const uint64_t t0 = wall_clock64();
-    phase_distances(V, S, sh, t, src, p.delta, p.err, ct);
+    phase_distances(V, S, sh, t, src, p.delta, 0, p.err, ct);
     const uint64_t t1 = wall_clock64();
-    const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.err, ct);
+    const uint32_t nlev = phase_sigma(V, S, sh, t, src, 0, p.err, ct);
     const uint64_t t2 = wall_clock64();
     phase_dependencies<MODE, 0, true>(p, V, S, t, task, 0, src, nlev);
     if (t.leader()) {

@@ -26,6 +26,7 @@ struct fbc_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
+    uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
     int cu_count = 0;
     size_t total_mem = 0;
     uint8_t *scratch = nullptr;
@@ -178,6 +179,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     p.scratch = ctx->scratch;
     p.slot_bytes = fbcdev::slot_bytes(p.nmax);
     p.delta = auto_delta(ctx, g);
+    p.sort_cap = ctx->sort_cap;
     p.queue_head = ctx->d_queue;
     p.err = ctx->d_err;
     p.te = ctx->d_te;
@@ -274,6 +276,18 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots)
     if (!ctx) return FBC_E_ARG;
     ctx->delta_override = bucket_delta;
     ctx->slots_override = slots;
+    return 0;
+}
+
+int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
+{
+    if (!ctx || !name) return FBC_E_ARG;
+    if (value < 0 || value > 0xFFFFFFFFll) return fail(ctx, FBC_E_ARG, "option value out of range");
+    const uint32_t v = (uint32_t)value;
+    if (!std::strcmp(name, "delta")) ctx->delta_override = v;
+    else if (!std::strcmp(name, "slots")) ctx->slots_override = v;
+    else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
+    else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
     return 0;
 }
 

@@ -67,7 +67,11 @@ struct BrandesParams {
     unsigned long long *te;   /* traversed-edge counter */
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
     uint32_t *grid_ctl;       /* plain kernel: traversal counters shared by the grid (16 words) */
+    uint32_t sort_cap;        /* batched kernel: sort near queues / levels of up to this many vertices (0: off) */
 };
+
+/* Largest near queue or level the batched kernel sorts (LDS words). */
+constexpr uint32_t SORT_CAP_MAX = 1024;
 
 /* Work counters of the persistent kernel (fbc_last_stats_ex). */
 enum Stat : int {

@@ -24,7 +24,7 @@ EXPORTS = (
     "fbc_graph_free", "fbc_sssp_sigma", "fbc_single_source_dep", "fbc_dep_batch",
     "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
-    "fbc_last_counters", "fbc_kmeans_batch", "fbc_pivot_phase_plain",
+    "fbc_last_counters", "fbc_kmeans_batch", "fbc_pivot_phase_plain", "fbc_set_option",
 )
 
 # Every symbol include/fbc_host.h declares.
@@ -83,6 +83,7 @@ def lib():
         L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
                                       ctypes.c_int, _f64p]
         L.fbc_pivot_phase_plain.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p]
+        L.fbc_set_option.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int64]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
         L.fbc_kmeans_batch.argtypes = [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _f64p,
@@ -134,6 +135,10 @@ class Context:
 
     def set_tuning(self, bucket_delta=0, slots=0):
         self.check(lib().fbc_set_tuning(self.h, bucket_delta, slots))
+
+    def set_option(self, name, value):
+        """Named tuning option: "delta", "slots", "frontier_sort"."""
+        self.check(lib().fbc_set_option(self.h, name.encode(), int(value)))
 
     def last_stats(self):
         ms, te = ctypes.c_double(), ctypes.c_double()

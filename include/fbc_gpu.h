@@ -69,6 +69,12 @@ const char *fbc_last_error(const fbc_ctx *ctx);
  * slots bounds the sources in flight (one workgroup each). */
 int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
 
+/* Named tuning option (0 = automatic / off): "delta", "slots" (as above),
+ * "frontier_sort" (the batched kernel sorts near queues and DAG levels of at
+ * most this many vertices, up to 1024, before expanding them).  No
+ * counterpart in the reference. */
+int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
+
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
  * Replaces the device-side use of DirectedWeightedGraph's adjacency maps
  * (DirectedWeightedGraph.h:57-58, forwardStar :111-115). */

@@ -96,6 +96,21 @@ def test_cfg0_exact_bc(cfg0):
         assert (np.abs(bc - r2) / np.maximum(np.abs(r2), 1e-12 * np.abs(r2).max())).max() <= 1e-9
 
 
+def test_cfg0_sorted_frontier_bitexact(cfg0, gpu_ctx):
+    """Sorting the near queues and DAG levels changes only the order in which
+    vertices are expanded: distances, sigma and FP64 dependencies stay exact."""
+    g, og = cfg0
+    srcs = np.random.default_rng(5).choice(g.n, 16, replace=False)
+    gpu_ctx.set_option("frontier_sort", 1024)
+    try:
+        check_sssp(g, og, srcs[:8])
+        dep = g.dep_batch(srcs)
+    finally:
+        gpu_ctx.set_option("frontier_sort", 0)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(dep[i], og.single_source_dep(int(s))), f"source {s}"
+
+
 def test_cfg0_dep_f32c(cfg0):
     g, _ = cfg0
     srcs = np.arange(0, g.n, 257)
