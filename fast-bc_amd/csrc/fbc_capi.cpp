@@ -862,9 +862,10 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     std::memset(&p, 0, sizeof(p));
     p.ncl = nsets;
     p.min_variance = min_variance;
-    uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtiles = nullptr, *dtcl = nullptr, *dtp0 = nullptr;
+    uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtiles = nullptr;
     uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr, *dgo = nullptr;
-    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmean = nullptr, *dG = nullptr;
+    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmsum = nullptr, *dG = nullptr,
+           *dkv = nullptr;
     int32_t *dW = nullptr, *dcw = nullptr;
     uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
              *dmem = nullptr, *dact = nullptr;
@@ -878,7 +879,8 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         (e = upload_n(&dX, features, xs, ctx->stream)) != hipSuccess || (e = dalloc(&dxn, ps)) != hipSuccess ||
         (e = upload_n(&dW, weights, ps, ctx->stream)) != hipSuccess || (e = upload(&dact, active, ctx->stream)) != hipSuccess ||
         (e = dalloc(&dcd, ps)) != hipSuccess || (e = dalloc(&dmd, ps)) != hipSuccess ||
-        (e = dalloc(&dmean, std::max<uint64_t>(ms, 1))) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
+        (e = dalloc(&dkv, ks)) != hipSuccess ||
+        (e = dalloc(&dmsum, std::max<uint64_t>(ms, 1))) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
         (e = dalloc(&dcent, ks)) != hipSuccess || (e = dalloc(&dnext, ks)) != hipSuccess ||
         (e = dalloc(&dcnt, ks)) != hipSuccess || (e = dalloc(&dstart, ks)) != hipSuccess ||
         (e = dalloc(&dasg, ps)) != hipSuccess || (e = dalloc(&dmem, ps)) != hipSuccess ||
@@ -899,18 +901,20 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         p.weights = dW;
         p.cdist = dcd;
         p.mdist = dmd;
+        p.kvar = dkv;
         p.cent = dcent;
         p.next = dnext;
         p.cnt = dcnt;
         p.start = dstart;
         p.assign = dasg;
         p.members = dmem;
-        p.mean = dmean;
+        p.msum = dmsum;
         p.cweights = dcw;
         p.active = dact;
         (void)hipEventRecord(ctx->ev0, ctx->stream);
         for (const std::vector<uint32_t> &grp : groups) {
-            std::vector<uint32_t> tiles, tcl, tp0;
+            /* Gram tiles, then the iteration tiles, in one device array */
+            std::vector<uint32_t> tiles, pc, pf, sum, mc, mf, kc, kf;
             std::fill(active.begin(), active.end(), 0u);
             for (uint32_t s : grp) {
                 active[s] = 1;
@@ -922,22 +926,50 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
                         tiles.push_back(tj);
                     }
                 for (uint32_t p0 = 0; p0 < P[s]; p0 += 256) {
-                    tcl.push_back(s);
-                    tp0.push_back(p0);
+                    pc.push_back(s);
+                    pf.push_back(p0);
+                }
+                for (uint32_t m0 = 0; m0 < P[s]; m0 += 64) {
+                    mc.push_back(s);
+                    mf.push_back(m0);
+                    for (uint32_t f0 = 0; f0 < D[s]; f0 += 256) {
+                        sum.push_back(s);
+                        sum.push_back(m0);
+                        sum.push_back(f0);
+                    }
+                }
+                for (uint32_t k0 = 0; k0 < K[s]; k0 += 256) {
+                    kc.push_back(s);
+                    kf.push_back(k0);
                 }
             }
+            std::vector<uint32_t> all;
+            std::vector<size_t> at;
+            for (const std::vector<uint32_t> *v : {&tiles, &pc, &pf, &sum, &mc, &mf, &kc, &kf}) {
+                at.push_back(all.size());
+                all.insert(all.end(), v->begin(), v->end());
+            }
             dfree(dtiles);
-            dfree(dtcl);
-            dfree(dtp0);
-            if ((e = upload(&dtiles, tiles, ctx->stream)) != hipSuccess || (e = upload(&dtcl, tcl, ctx->stream)) != hipSuccess ||
-                (e = upload(&dtp0, tp0, ctx->stream)) != hipSuccess ||
+            if ((e = upload(&dtiles, all, ctx->stream)) != hipSuccess ||
                 (e = hipMemcpyAsync(dact, active.data(), nsets * sizeof(uint32_t), hipMemcpyHostToDevice,
                                     ctx->stream)) != hipSuccess ||
-                (e = fbcdev::km_launch_gram(p, dG, (uint32_t)(tiles.size() / 3), dtiles, ctx->stream)) != hipSuccess ||
+                (e = fbcdev::km_launch_gram(p, dG, (uint32_t)(tiles.size() / 3), dtiles + at[0], ctx->stream)) != hipSuccess ||
                 (e = fbcdev::km_launch_init(p, ctx->stream)) != hipSuccess)
                 break;
+            fbcdev::KmTiles kt;
+            kt.npt = (uint32_t)pc.size();
+            kt.nsum = (uint32_t)(sum.size() / 3);
+            kt.nmt = (uint32_t)mc.size();
+            kt.nkt = (uint32_t)kc.size();
+            kt.pt_cl = dtiles + at[1];
+            kt.pt_first = dtiles + at[2];
+            kt.sum = dtiles + at[3];
+            kt.mt_cl = dtiles + at[4];
+            kt.mt_first = dtiles + at[5];
+            kt.kt_cl = dtiles + at[6];
+            kt.kt_first = dtiles + at[7];
             for (uint32_t it = 1; it <= max_iteration + 1; ++it) {
-                e = fbcdev::km_launch_iteration(p, (uint32_t)tcl.size(), dtcl, dtp0, it, max_iteration, ctx->stream);
+                e = fbcdev::km_launch_iteration(p, kt, it, max_iteration, ctx->stream);
                 if (e != hipSuccess) break;
                 e = hipMemcpyAsync(active.data(), dact, nsets * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -957,9 +989,9 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         if (e == hipSuccess) e = hipMemcpy(cweights, dcw, ks * sizeof(int32_t), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "k-means run");
     }
-    for (auto *q : {dP, dD, dK, dtiles, dtcl, dtp0, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
+    for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
     for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
-    for (auto *q : {dX, dxn, dcd, dmd, dmean, dG}) dfree(q);
+    for (auto *q : {dX, dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
     dfree(dW);
     dfree(dcw);
     ctx->last_ms = ms_total;

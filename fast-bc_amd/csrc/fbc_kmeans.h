@@ -16,7 +16,7 @@ struct KmParams {
     const uint64_t *xoff;               /* row block of the set in X (P * D doubles) */
     const uint64_t *poff;               /* per-point arrays */
     const uint64_t *koff;               /* per-centroid arrays */
-    const uint64_t *moff;               /* mean block (K * D doubles) */
+    const uint64_t *moff;               /* member-sum block (K * D doubles) */
     const uint64_t *goff;               /* Gram block of the set in G (P * P doubles) */
     const double *X;                    /* features, row-major */
     double *xnorm;                      /* squared norm of every row (exact; the Gram diagonal) */
@@ -24,9 +24,10 @@ struct KmParams {
     const int32_t *weights;             /* per point */
     double *cdist;                      /* per point, initialisation */
     double *mdist;                      /* per member slot: distance to its centroid's mean */
+    double *kvar;                       /* per centroid: distance from its centroid to the next */
     uint32_t *cent, *next, *cnt, *start;/* per centroid */
     uint32_t *assign, *members;         /* per point */
-    double *mean;
+    double *msum;                       /* per centroid and feature: sum over members */
     int32_t *cweights;                  /* per centroid, final */
     uint32_t *active;                   /* per set: 1 while iterating */
     double min_variance;
@@ -40,10 +41,17 @@ constexpr uint32_t KM_MAX_K = 12288;
 hipError_t km_launch_gram(const KmParams &p, double *G, uint32_t ntiles, const uint32_t *tiles,
                           hipStream_t s);
 hipError_t km_launch_init(const KmParams &p, hipStream_t s);
-/* point tiles: (set, first point) pairs of 256 points */
-hipError_t km_launch_iteration(const KmParams &p, uint32_t ntiles, const uint32_t *tile_cl,
-                               const uint32_t *tile_p0, uint32_t iteration, uint32_t max_iteration,
-                               hipStream_t s);
+/* Work tiles of one iteration (device arrays, built by the host per group):
+ *   pt   (set, first point) of 256 points       -- assignment
+ *   sum  (set, first slot, first feature) of 64 member slots x 256 features
+ *   mt   (set, first slot) of 64 member slots    -- member distances
+ *   kt   (set, first centroid) of 256 centroids  -- nearest members */
+struct KmTiles {
+    uint32_t npt, nsum, nmt, nkt;
+    const uint32_t *pt_cl, *pt_first, *sum, *mt_cl, *mt_first, *kt_cl, *kt_first;
+};
+hipError_t km_launch_iteration(const KmParams &p, const KmTiles &t, uint32_t iteration,
+                               uint32_t max_iteration, hipStream_t s);
 
 } // namespace fbcdev
 

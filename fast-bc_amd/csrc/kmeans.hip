@@ -22,9 +22,13 @@
  *                one 1024-thread workgroup per set
  *   k_km_assign  first-wins nearest centroid of every point, 256 points per
  *                workgroup
- *   k_km_update  members, means, nearest member to each mean, the variance
- *                test (PlusPlusKMeans.h:115-151) and the final weights, one
- *                1024-thread workgroup per set
+ *   k_km_members member lists grouped by centroid, one 1024-thread
+ *                workgroup per set
+ *   k_km_msum    member sums per centroid (the means' numerators), coalesced
+ *   k_km_dist    member-to-mean distances (PlusPlusKMeans.h:130-146)
+ *   k_km_pick    nearest member to each mean
+ *   k_km_finish  the variance test (PlusPlusKMeans.h:150-151), then the
+ *                final weights or the next centroids
  */
 #include "fbc_kmeans.h"
 
@@ -36,6 +40,8 @@ constexpr int GD = 16;      /* features per LDS stage */
 constexpr int SWG = 1024;   /* per-set workgroups */
 constexpr int NWAVE = SWG / 64;
 constexpr int AWG = 256;    /* points per assignment workgroup */
+constexpr int SNAP_M = 64;  /* members per LDS stage of the snap */
+constexpr int SNAP_F = 64;  /* features per LDS stage of the snap */
 
 /* sum over features of (a - b)^2 in feature order */
 __device__ __forceinline__ double sq_rows(const double *a, const double *b, uint32_t D)
@@ -238,29 +244,21 @@ __global__ __launch_bounds__(AWG) void k_km_assign(KmParams p, const uint32_t *t
     if (valid) p.assign[p.poff[c] + v] = bk;
 }
 
-__global__ __launch_bounds__(SWG) void k_km_update(KmParams p, uint32_t iteration, uint32_t max_iteration)
+/* Member lists grouped by centroid (one workgroup per set).  The order inside
+ * a group is irrelevant: sums are exact and the nearest member is chosen by
+ * (distance, index). */
+__global__ __launch_bounds__(SWG) void k_km_members(KmParams p)
 {
     const uint32_t c = blockIdx.x;
     if (!p.active[c]) return;
-    const uint32_t P = p.P[c], D = p.D[c], K = p.K[c];
-    const double *X = p.X + p.xoff[c];
-    const double *xn = p.xnorm + p.poff[c];
-    const double *Gs = p.G + p.goff[c];
+    const uint32_t P = p.P[c], K = p.K[c];
     const uint32_t *asg = p.assign + p.poff[c];
-    uint32_t *cent = p.cent + p.koff[c];
-    uint32_t *next = p.next + p.koff[c];
     uint32_t *cnt = p.cnt + p.koff[c];
     uint32_t *start = p.start + p.koff[c];
     uint32_t *members = p.members + p.poff[c];
-    double *md = p.mdist + p.poff[c];
-    double *mean = p.mean + p.moff[c];
     __shared__ uint32_t s_cnt[KM_MAX_K];
     __shared__ uint32_t s_start[KM_MAX_K];
     __shared__ uint32_t s_wave[NWAVE];
-    __shared__ double s_var[NWAVE];
-
-    /* members grouped by centroid (the order inside a group is irrelevant:
-     * sums are exact and the nearest member is chosen by (distance, index)) */
     for (uint32_t k = threadIdx.x; k < K; k += SWG) s_cnt[k] = 0;
     __syncthreads();
     for (uint32_t v = threadIdx.x; v < P; v += SWG) atomicAdd(&s_cnt[asg[v]], 1u);
@@ -285,45 +283,141 @@ __global__ __launch_bounds__(SWG) void k_km_update(KmParams p, uint32_t iteratio
         const uint32_t k = asg[v];
         members[s_start[k] + atomicAdd(&s_cnt[k], 1u)] = v;
     }
-    __syncthreads();
-    /* means: exact integer sums divided once by the member count */
-    for (uint64_t e = threadIdx.x; e < (uint64_t)K * D; e += SWG) {
-        const uint32_t k = (uint32_t)(e / D), f = (uint32_t)(e % D);
-        const uint32_t n = s_cnt[k];
-        if (!n) continue;
-        double s = 0;
-        for (uint32_t i = 0; i < n; ++i) s += X[(uint64_t)members[s_start[k] + i] * D + f];
-        mean[e] = s / (double)n;
+    /* member sums start from zero */
+    double *msum = p.msum + p.moff[c];
+    for (uint64_t e = threadIdx.x; e < (uint64_t)K * p.D[c]; e += SWG) msum[e] = 0.0;
+}
+
+/* Member sums per centroid: a workgroup takes 64 member slots (grouped by
+ * centroid) x 256 features; each thread walks the slots of its feature with
+ * coalesced row loads and adds every run of one centroid into msum with an
+ * FP64 atomic.  The sums are integers below 2^53, so any order is exact.
+ * grid: (set, first slot, first feature) triples. */
+__global__ __launch_bounds__(AWG) void k_km_msum(KmParams p, const uint32_t *tiles)
+{
+    const uint32_t c = tiles[3 * blockIdx.x];
+    if (!p.active[c]) return;
+    const uint32_t m0 = tiles[3 * blockIdx.x + 1], f = tiles[3 * blockIdx.x + 2] + threadIdx.x;
+    const uint32_t P = p.P[c], D = p.D[c];
+    const double *X = p.X + p.xoff[c];
+    const uint32_t *asg = p.assign + p.poff[c];
+    const uint32_t *members = p.members + p.poff[c];
+    double *msum = p.msum + p.moff[c];
+    __shared__ uint32_t s_v[SNAP_M], s_k[SNAP_M];
+    const uint32_t nm = P - m0 < (uint32_t)SNAP_M ? P - m0 : (uint32_t)SNAP_M;
+    if (threadIdx.x < nm) {
+        const uint32_t v = members[m0 + threadIdx.x];
+        s_v[threadIdx.x] = v;
+        s_k[threadIdx.x] = asg[v];
     }
     __syncthreads();
-    /* distance of every member to its centroid's mean, reference order */
-    for (uint32_t i = threadIdx.x; i < P; i += SWG) {
-        const uint32_t v = members[i];
-        md[i] = sq_rows(mean + (uint64_t)asg[v] * D, X + (uint64_t)v * D, D);
+    if (f >= D) return;
+    double acc = 0;
+    uint32_t k = s_k[0];
+    for (uint32_t i = 0; i < nm; ++i) {
+        if (s_k[i] != k) {
+            unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
+            acc = 0;
+            k = s_k[i];
+        }
+        acc += X[(uint64_t)s_v[i] * D + f];
+    }
+    unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
+}
+
+/* Distance of 64 member slots to their centroid's mean (sum / count, the
+ * reference's mean), summed in the reference's sequential feature order by
+ * one thread per slot; differences of 64 slots x 64 features are staged
+ * through LDS with coalesced row loads.  grid: (set, first slot) pairs. */
+__global__ __launch_bounds__(AWG) void k_km_dist(KmParams p, const uint32_t *tile_cl, const uint32_t *tile_m0)
+{
+    const uint32_t c = tile_cl[blockIdx.x];
+    if (!p.active[c]) return;
+    const uint32_t m0 = tile_m0[blockIdx.x];
+    const uint32_t P = p.P[c], D = p.D[c];
+    const double *X = p.X + p.xoff[c];
+    const uint32_t *asg = p.assign + p.poff[c];
+    const uint32_t *cnt = p.cnt + p.koff[c];
+    const uint32_t *members = p.members + p.poff[c];
+    const double *msum = p.msum + p.moff[c];
+    double *md = p.mdist + p.poff[c];
+    __shared__ double s_d[SNAP_M][SNAP_F + 1];
+    __shared__ uint32_t s_v[SNAP_M], s_k[SNAP_M];
+    __shared__ double s_n[SNAP_M];
+    const uint32_t nm = P - m0 < (uint32_t)SNAP_M ? P - m0 : (uint32_t)SNAP_M;
+    if (threadIdx.x < nm) {
+        const uint32_t v = members[m0 + threadIdx.x], k = asg[v];
+        s_v[threadIdx.x] = v;
+        s_k[threadIdx.x] = k;
+        s_n[threadIdx.x] = (double)cnt[k];
     }
     __syncthreads();
-    /* nearest member to each mean: smallest distance, then smallest index
-     * (the reference scans members in point order with a strict '<') */
-    double var = 0;
-    for (uint32_t k = threadIdx.x; k < K; k += SWG) {
-        const uint32_t n = s_cnt[k], b0 = s_start[k];
-        uint32_t bv = cent[k];
-        if (n) {
-            bv = members[b0];
-            double bd = md[b0];
-            for (uint32_t i = 1; i < n; ++i) {
-                const uint32_t v = members[b0 + i];
-                const double d = md[b0 + i];
-                if (d < bd || (d == bd && v < bv)) {
-                    bv = v;
-                    bd = d;
-                }
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    double sq = 0;
+    for (uint32_t f0 = 0; f0 < D; f0 += SNAP_F) {
+        const uint32_t f = f0 + lane;
+        if (f < D) {
+#pragma unroll 4
+            for (uint32_t mi = wave; mi < nm; mi += AWG / 64)
+                s_d[mi][lane] = msum[(uint64_t)s_k[mi] * D + f] / s_n[mi] - X[(uint64_t)s_v[mi] * D + f];
+        }
+        __syncthreads();
+        if (threadIdx.x < nm) {
+            const uint32_t nf = D - f0 < (uint32_t)SNAP_F ? D - f0 : (uint32_t)SNAP_F;
+            for (uint32_t j = 0; j < nf; ++j) {
+                const double d = s_d[threadIdx.x][j];
+                sq += d * d;
             }
         }
-        next[k] = bv;
-        const double d = gram_dist(Gs, xn, P, cent[k], bv);
-        if (d > var) var = d;
+        __syncthreads();
     }
+    if (threadIdx.x < nm) md[m0 + threadIdx.x] = sq;
+}
+
+/* Nearest member to each mean -- smallest distance, then smallest index (the
+ * reference scans members in point order with a strict '<') -- and the
+ * distance from the current centroid to it.  grid: (set, first centroid). */
+__global__ __launch_bounds__(AWG) void k_km_pick(KmParams p, const uint32_t *tile_cl, const uint32_t *tile_k0)
+{
+    const uint32_t c = tile_cl[blockIdx.x];
+    if (!p.active[c]) return;
+    const uint32_t k = tile_k0[blockIdx.x] + threadIdx.x;
+    const uint32_t P = p.P[c], K = p.K[c];
+    if (k >= K) return;
+    const double *xn = p.xnorm + p.poff[c];
+    const double *Gs = p.G + p.goff[c];
+    const uint32_t *cent = p.cent + p.koff[c];
+    const uint32_t n = p.cnt[p.koff[c] + k], b0 = p.start[p.koff[c] + k];
+    const uint32_t *members = p.members + p.poff[c];
+    const double *md = p.mdist + p.poff[c];
+    uint32_t bv = cent[k];
+    if (n) {
+        bv = members[b0];
+        double bd = md[b0];
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint32_t v = members[b0 + i];
+            const double d = md[b0 + i];
+            if (d < bd || (d == bd && v < bv)) {
+                bv = v;
+                bd = d;
+            }
+        }
+    }
+    p.next[p.koff[c] + k] = bv;
+    p.kvar[p.koff[c] + k] = gram_dist(Gs, xn, P, cent[k], bv);
+}
+
+/* The variance test (PlusPlusKMeans.h:150-151), then the final weights or the
+ * next centroids (one workgroup per set). */
+__global__ __launch_bounds__(AWG) void k_km_finish(KmParams p, uint32_t iteration, uint32_t max_iteration)
+{
+    const uint32_t c = blockIdx.x;
+    if (!p.active[c]) return;
+    const uint32_t K = p.K[c];
+    const double *kvar = p.kvar + p.koff[c];
+    __shared__ double s_var[AWG / 64];
+    double var = 0;
+    for (uint32_t k = threadIdx.x; k < K; k += AWG) var = kvar[k] > var ? kvar[k] : var;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const double ov = __shfl_xor(var, o, 64);
@@ -332,21 +426,26 @@ __global__ __launch_bounds__(SWG) void k_km_update(KmParams p, uint32_t iteratio
     if (threadIdx.x % 64 == 0) s_var[threadIdx.x / 64] = var;
     __syncthreads();
     double vmax = s_var[0];
-    for (int w = 1; w < NWAVE; ++w) vmax = s_var[w] > vmax ? s_var[w] : vmax;
+    for (int w = 1; w < AWG / 64; ++w) vmax = s_var[w] > vmax ? s_var[w] : vmax;
     const bool go_on = vmax > p.min_variance && iteration <= max_iteration;
+    uint32_t *cent = p.cent + p.koff[c];
+    const uint32_t *next = p.next + p.koff[c];
     if (!go_on) {
         /* final: the centroids of this assignment and their members' weights */
         const int32_t *w = p.weights + p.poff[c];
+        const uint32_t *cnt = p.cnt + p.koff[c];
+        const uint32_t *start = p.start + p.koff[c];
+        const uint32_t *members = p.members + p.poff[c];
         int32_t *cw = p.cweights + p.koff[c];
-        for (uint32_t k = threadIdx.x; k < K; k += SWG) {
+        for (uint32_t k = threadIdx.x; k < K; k += AWG) {
             int32_t s = 0;
-            for (uint32_t i = 0; i < s_cnt[k]; ++i) s += w[members[s_start[k] + i]];
+            for (uint32_t i = 0; i < cnt[k]; ++i) s += w[members[start[k] + i]];
             cw[k] = s;
         }
+        __syncthreads();
         if (threadIdx.x == 0) p.active[c] = 0;
     } else {
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < K; k += SWG) cent[k] = next[k];
+        for (uint32_t k = threadIdx.x; k < K; k += AWG) cent[k] = next[k];
     }
 }
 
@@ -363,14 +462,21 @@ hipError_t km_launch_init(const KmParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t km_launch_iteration(const KmParams &p, uint32_t ntiles, const uint32_t *tile_cl,
-                               const uint32_t *tile_p0, uint32_t iteration, uint32_t max_iteration,
-                               hipStream_t s)
+hipError_t km_launch_iteration(const KmParams &p, const KmTiles &t, uint32_t iteration,
+                               uint32_t max_iteration, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_km_assign, dim3(ntiles), dim3(AWG), 0, s, p, tile_cl, tile_p0);
+    hipLaunchKernelGGL(k_km_assign, dim3(t.npt), dim3(AWG), 0, s, p, t.pt_cl, t.pt_first);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_km_update, dim3(p.ncl), dim3(SWG), 0, s, p, iteration, max_iteration);
+    hipLaunchKernelGGL(k_km_members, dim3(p.ncl), dim3(SWG), 0, s, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_km_msum, dim3(t.nsum), dim3(AWG), 0, s, p, t.sum);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_km_dist, dim3(t.nmt), dim3(AWG), 0, s, p, t.mt_cl, t.mt_first);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_km_pick, dim3(t.nkt), dim3(AWG), 0, s, p, t.kt_cl, t.kt_first);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_km_finish, dim3(p.ncl), dim3(AWG), 0, s, p, iteration, max_iteration);
     return hipGetLastError();
 }
 
