@@ -210,7 +210,7 @@ struct Counters {
 /* ---- execution teams ------------------------------------------------------ */
 
 /* Counters of one source's traversal shared by all threads of its team: in
- * LDS for a workgroup team, in device memory for the grid team. */
+ * LDS for a workgroup or wavefront team, in device memory for the grid team. */
 struct Ctl {
     uint32_t qcnt[3];     /* near-queue lengths by round mod 3 */
     uint32_t fcnt[2];     /* far-list lengths by window parity */
@@ -218,14 +218,27 @@ struct Ctl {
     uint32_t tail;        /* level-list length */
 };
 
-/* Per-workgroup LDS. */
-struct Shared {
-    Ctl ctl;
-    uint32_t task;
+/* LDS of one local group (a workgroup, or one wavefront of a wavefront team):
+ * the vertices of degree > HEAVY_DEG met in the current round or level, whose
+ * arcs the group's threads split, and the traversed-edge count. */
+struct Local {
     uint32_t nheavy[2];   /* by round (level) parity */
     uint32_t heavy[HEAVY_CAP];
     unsigned long long te;
+};
+
+/* Per-workgroup LDS of the workgroup-per-source and grid kernels. */
+struct Shared {
+    Ctl ctl;
+    Local loc;
+    uint32_t task;
     uint32_t sortbuf[SORT_CAP_MAX];
+};
+
+/* Per-workgroup LDS of the wavefront-per-source kernel. */
+struct SharedWave {
+    Ctl ctl[WAVES];
+    Local loc[WAVES];
 };
 
 /* Sort q[0, n) ascending through LDS (bitonic; n <= SORT_CAP_MAX), so that a
@@ -256,15 +269,51 @@ __device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
 }
 
 
+/* A team runs one source.  rank/size/leader/sync span the team; the l*
+ * members span its local group, which shares one Local in LDS. */
+
 /* One workgroup per source (batched kernel). */
 struct TeamWG {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
     Ctl *ctl;
+    Local *loc;
+    uint32_t *sortbuf;
     __device__ uint32_t rank() const { return threadIdx.x; }
     __device__ uint32_t size() const { return WG; }
     __device__ bool leader() const { return threadIdx.x == 0; }
     __device__ void sync() const { __syncthreads(); }
     __device__ bool aborted() const { return false; }
+    __device__ uint32_t lrank() const { return threadIdx.x; }
+    __device__ uint32_t lsize() const { return WG; }
+    __device__ bool lleader() const { return threadIdx.x == 0; }
+    __device__ void lsync() const { __syncthreads(); }
+};
+
+/* Barrier of one wavefront: its lanes' memory operations before it are
+ * visible to its lanes after it. */
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+/* One wavefront per source (small views): WAVES sources per workgroup, no
+ * workgroup barriers.  Slot state keeps workgroup scope (the wavefront is
+ * part of one workgroup). */
+struct TeamWave {
+    static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
+    Ctl *ctl;
+    Local *loc;
+    __device__ uint32_t rank() const { return threadIdx.x & 63; }
+    __device__ uint32_t size() const { return 64; }
+    __device__ bool leader() const { return (threadIdx.x & 63) == 0; }
+    __device__ void sync() const { wave_sync(); }
+    __device__ bool aborted() const { return false; }
+    __device__ uint32_t lrank() const { return threadIdx.x & 63; }
+    __device__ uint32_t lsize() const { return 64; }
+    __device__ bool lleader() const { return (threadIdx.x & 63) == 0; }
+    __device__ void lsync() const { wave_sync(); }
 };
 
 /* The whole co-resident grid per source (plain kernel, launched with
@@ -273,11 +322,16 @@ struct TeamWG {
 struct TeamGrid {
     static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
     Ctl *ctl;
+    Local *loc;
     __device__ uint32_t rank() const { return blockIdx.x * WG + threadIdx.x; }
     __device__ uint32_t size() const { return gridDim.x * WG; }
     __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
     __device__ void sync() const { cooperative_groups::this_grid().sync(); }
     __device__ bool aborted() const { return false; }
+    __device__ uint32_t lrank() const { return threadIdx.x; }
+    __device__ uint32_t lsize() const { return WG; }
+    __device__ bool lleader() const { return threadIdx.x == 0; }
+    __device__ void lsync() const { __syncthreads(); }
 };
 
 template <class T> struct IsWG { static constexpr bool value = false; };
@@ -373,7 +427,7 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
  * read two rounds earlier, so one team barrier per round suffices.  Far-list
  * counters and window minima alternate by window parity in the same way. */
 template <class T>
-__device__ __forceinline__ void phase_distances(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
+__device__ __forceinline__ void phase_distances(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t delta, uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
@@ -394,7 +448,8 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, Sha
         ast32<SC>(&c.minv[1], DINF);
         S.qa[0] = src;
     }
-    if (threadIdx.x == 0) sh.nheavy[0] = sh.nheavy[1] = 0;
+    Local &L = *t.loc;
+    if (t.lleader()) L.nheavy[0] = L.nheavy[1] = 0;
     t.sync();
 
     uint32_t *qa = S.qa, *qb = S.qb, *fa = S.fa, *fb = S.fb;
@@ -405,13 +460,14 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, Sha
             const uint32_t nxt = rnd + 1;
             uint32_t *qn_cnt = &c.qcnt[nxt % 3];
             uint32_t *far_cnt = &c.fcnt[win & 1];
-            uint32_t *nh_cnt = &sh.nheavy[rnd & 1];
+            uint32_t *nh_cnt = &L.nheavy[rnd & 1];
             if (t.leader()) {
                 ++ct.c[STAT_ROUNDS];
                 ast32<SC>(&c.qcnt[(rnd + 2) % 3], 0u);
             }
-            if (threadIdx.x == 0) sh.nheavy[(rnd + 1) & 1] = 0;
-            if (IsWG<T>::value && near > 64 && near <= sort_cap) sort_vertices(qa, near, sh.sortbuf);
+            if (t.lleader()) L.nheavy[(rnd + 1) & 1] = 0;
+            if constexpr (IsWG<T>::value)
+                if (near > 64 && near <= sort_cap) sort_vertices(qa, near, t.sortbuf);
             for (uint32_t i = r0; i < near; i += R) {
                 const uint32_t u = qa[i];
                 ++ct.c[STAT_VISITS];
@@ -420,19 +476,19 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, Sha
                 if (e1 - e0 > (uint32_t)HEAVY_DEG) {
                     const uint32_t h = atomicAdd(nh_cnt, 1u);
                     if (h < (uint32_t)HEAVY_CAP) {
-                        sh.heavy[h] = u;
+                        L.heavy[h] = u;
                         continue;
                     }
                 }
                 relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
             }
-            __syncthreads();
+            t.lsync();
             const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
-            for (uint32_t h = 0; h < nh; ++h) {   /* heavy vertices: the workgroup splits the arcs */
-                const uint32_t u = sh.heavy[h];
+            for (uint32_t h = 0; h < nh; ++h) {   /* heavy vertices: the local group splits the arcs */
+                const uint32_t u = L.heavy[h];
                 const uint32_t du = dist_of<SC>(S, u);
                 const uint32_t e1 = V.rp[u + 1];
-                for (uint32_t e = V.rp[u] + threadIdx.x; e < e1; e += WG)
+                for (uint32_t e = V.rp[u] + t.lrank(); e < e1; e += t.lsize())
                     relax_arc<T>(V, S, du, e, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
             }
             t.sync();
@@ -521,7 +577,7 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
  * in-degree goes into the low half of key[v] (the round tags are done with);
  * the high half keeps the distance. */
 template <class T>
-__device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Shared &sh, const T &t, uint32_t src,
+__device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
@@ -549,7 +605,8 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Sha
         }
         ast64<SC>(&S.key[v], ((uint64_t)dv << 32) | cin);
     }
-    if (te) atomicAdd(&sh.te, te);
+    Local &L = *t.loc;
+    if (te) atomicAdd(&L.te, te);
     if (t.leader()) {
         S.sd[2 * (uint64_t)src] = 1;
         S.qa[0] = src;
@@ -557,31 +614,32 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, Sha
         S.fa[1] = 1;
         ast32<SC>(&c.tail, 1u);
     }
-    if (threadIdx.x == 0) sh.nheavy[0] = sh.nheavy[1] = 0;
+    if (t.lleader()) L.nheavy[0] = L.nheavy[1] = 0;
     t.sync();
     uint32_t lb = 0, le = 1, nlev = 1;
     for (uint32_t lvl = 0; lb < le; ++lvl) {
-        uint32_t *nh_cnt = &sh.nheavy[lvl & 1];
-        if (threadIdx.x == 0) sh.nheavy[(lvl + 1) & 1] = 0;
-        if (IsWG<T>::value && le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, sh.sortbuf);
+        uint32_t *nh_cnt = &L.nheavy[lvl & 1];
+        if (t.lleader()) L.nheavy[(lvl + 1) & 1] = 0;
+        if constexpr (IsWG<T>::value)
+            if (le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, t.sortbuf);
         for (uint32_t i = lb + r0; i < le; i += R) {
             const uint32_t u = S.qa[i];
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
             if (e1 - e0 > (uint32_t)HEAVY_DEG) {
                 const uint32_t h = atomicAdd(nh_cnt, 1u);
                 if (h < (uint32_t)HEAVY_CAP) {
-                    sh.heavy[h] = u;
+                    L.heavy[h] = u;
                     continue;
                 }
             }
             push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, &c.tail, err);
         }
-        __syncthreads();
+        t.lsync();
         const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
         for (uint32_t h = 0; h < nh; ++h) {
-            const uint32_t u = sh.heavy[h];
-            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + threadIdx.x,
-                          V.rp[u + 1], WG, &c.tail, err);
+            const uint32_t u = L.heavy[h];
+            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + t.lrank(),
+                          V.rp[u + 1], t.lsize(), &c.tail, err);
         }
         t.sync();
         lb = le;
@@ -746,87 +804,118 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
     }
 }
 
-__device__ __forceinline__ void flush_counters(const BrandesParams &p, Shared &sh, const Counters &ct)
+template <class T>
+__device__ __forceinline__ void flush_counters(const BrandesParams &p, const T &t, const Counters &ct)
 {
-    __syncthreads();
-    if (threadIdx.x == 0 && sh.te) atomicAdd(p.te, sh.te);
+    t.lsync();
+    if (t.lleader() && t.loc->te) atomicAdd(p.te, t.loc->te);
     if (p.stats)
         for (int i = 0; i < STAT_COUNT; ++i)
             if (ct.c[i]) atomicAdd(&p.stats[i], ct.c[i]);
 }
 
-/* ---- the batched kernel: one workgroup per source, device work queue ------- */
+/* One (view, source) task of the batched kernels, run by team t on slot S. */
+template <int MODE, int PREC, bool ORDERED, class T>
+__device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, const T &t, uint32_t task,
+                                         Counters &ct)
+{
+    View V;
+    uint32_t c = 0;
+    if (MODE == MODE_CLUSTER) {
+        c = p.task_aux[task];
+        const uint64_t ro = p.cl_rpoff[c];
+        V.n = p.cl_n[c];
+        V.rp = p.rp + ro;
+        V.rrp = p.rrp + ro;
+    } else {
+        V.n = p.n;
+        V.rp = p.rp;
+        V.rrp = p.rrp;
+    }
+    V.arcs = p.arcs;
+    V.rarcs = p.rarcs;
+    const uint32_t src = p.task_src[task];
+    const uint32_t r0 = t.rank(), R = t.size();
 
+    const uint64_t t0 = wall_clock64();
+    phase_distances(V, S, t, src, p.delta, p.sort_cap, p.err, ct);
+    const uint64_t t1 = wall_clock64();
+    const uint32_t nlev = phase_sigma(V, S, t, src, p.sort_cap, p.err, ct);
+    const uint64_t t2 = wall_clock64();
+    if (t.leader()) {
+        ct.c[STAT_T_DIST] += t1 - t0;
+        ct.c[STAT_T_SIGMA] += t2 - t1;
+    }
+
+    if (MODE == MODE_SSSP) {
+        uint64_t *od = p.out_dist + (uint64_t)task * V.n;
+        uint64_t *os = p.out_sigma + (uint64_t)task * V.n;
+        for (uint32_t v = r0; v < V.n; v += R) {
+            const uint32_t d = dist_final(S, v);
+            od[v] = d == DINF ? ~uint64_t(0) : d;
+            os[v] = S.sd[2 * (uint64_t)v];
+        }
+        t.sync();
+        return;
+    }
+    if (MODE == MODE_CLUSTER) {
+        const uint32_t b0 = p.cl_bptr[c], nb = p.cl_bptr[c + 1] - b0;
+        const uint64_t row = p.cl_vioff[c] + (uint64_t)src * nb;
+        for (uint32_t j = r0; j < nb; j += R) {
+            const uint32_t b = p.cl_blocal[b0 + j];
+            const uint32_t d = dist_final(S, b);
+            p.vi_len[row + j] = d == DINF ? 0 : d;
+            p.vi_cnt[row + j] = S.sd[2 * (uint64_t)b];
+        }
+    }
+    phase_dependencies<MODE, PREC, ORDERED>(p, V, S, t, task, c, src, nlev);
+    if (t.leader()) ct.c[STAT_T_DEP] += wall_clock64() - t2;
+}
+
+/* ---- the batched kernels: a device work queue of (view, source) tasks ------ */
+
+/* One workgroup per source. */
 template <int MODE, int PREC, bool ORDERED>
 __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
 {
     __shared__ Shared sh;
-    const int tid = threadIdx.x;
     Slot S;
     slot_layout(p.nmax, p.scratch + (uint64_t)blockIdx.x * p.slot_bytes, &S);
-    const TeamWG t{&sh.ctl};
-    if (tid == 0) sh.te = 0;
+    const TeamWG t{&sh.ctl, &sh.loc, sh.sortbuf};
+    if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;
-
     for (;;) {
-        if (tid == 0) sh.task = atomicAdd(p.queue_head, 1u);
+        if (threadIdx.x == 0) sh.task = atomicAdd(p.queue_head, 1u);
         __syncthreads();
         const uint32_t task = sh.task;
         __syncthreads();
         if (task >= p.ntasks) break;
-
-        View V;
-        uint32_t c = 0;
-        if (MODE == MODE_CLUSTER) {
-            c = p.task_aux[task];
-            const uint64_t ro = p.cl_rpoff[c];
-            V.n = p.cl_n[c];
-            V.rp = p.rp + ro;
-            V.rrp = p.rrp + ro;
-        } else {
-            V.n = p.n;
-            V.rp = p.rp;
-            V.rrp = p.rrp;
-        }
-        V.arcs = p.arcs;
-        V.rarcs = p.rarcs;
-        const uint32_t src = p.task_src[task];
-
-        const uint64_t t0 = wall_clock64();
-        phase_distances(V, S, sh, t, src, p.delta, p.sort_cap, p.err, ct);
-        const uint64_t t1 = wall_clock64();
-        const uint32_t nlev = phase_sigma(V, S, sh, t, src, p.sort_cap, p.err, ct);
-        const uint64_t t2 = wall_clock64();
-        if (tid == 0) {
-            ct.c[STAT_T_DIST] += t1 - t0;
-            ct.c[STAT_T_SIGMA] += t2 - t1;
-        }
-
-        if (MODE == MODE_SSSP) {
-            uint64_t *od = p.out_dist + (uint64_t)task * V.n;
-            uint64_t *os = p.out_sigma + (uint64_t)task * V.n;
-            for (uint32_t v = tid; v < V.n; v += WG) {
-                const uint32_t d = dist_final(S, v);
-                od[v] = d == DINF ? ~uint64_t(0) : d;
-                os[v] = S.sd[2 * (uint64_t)v];
-            }
-            __syncthreads();
-            continue;
-        }
-        if (MODE == MODE_CLUSTER) {
-            const uint32_t b0 = p.cl_bptr[c], nb = p.cl_bptr[c + 1] - b0;
-            const uint64_t row = p.cl_vioff[c] + (uint64_t)src * nb;
-            for (uint32_t j = tid; j < nb; j += WG) {
-                const uint32_t b = p.cl_blocal[b0 + j];
-                const uint32_t d = dist_final(S, b);
-                p.vi_len[row + j] = d == DINF ? 0 : d;
-                p.vi_cnt[row + j] = S.sd[2 * (uint64_t)b];
-            }
-        }
-        phase_dependencies<MODE, PREC, ORDERED>(p, V, S, t, task, c, src, nlev);
-        if (tid == 0) ct.c[STAT_T_DEP] += wall_clock64() - t2;
+        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
     }
-    flush_counters(p, sh, ct);
+    flush_counters(p, t, ct);
+}
+
+/* One wavefront per source: WAVES slots per workgroup, each wavefront pulls
+ * its own tasks. */
+template <int MODE, int PREC, bool ORDERED>
+__global__ __launch_bounds__(WG) void k_brandes_wave(BrandesParams p)
+{
+    __shared__ SharedWave sw;
+    const uint32_t w = threadIdx.x / 64;
+    if (blockIdx.x * WAVES + w >= p.nslots) return;   /* no slot behind this wavefront */
+    Slot S;
+    slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * WAVES + w) * p.slot_bytes, &S);
+    const TeamWave t{&sw.ctl[w], &sw.loc[w]};
+    if (t.lleader()) sw.loc[w].te = 0;
+    Counters ct;
+    for (;;) {
+        uint32_t task = 0;
+        if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
+        task = __shfl(task, 0, 64);
+        if (task >= p.ntasks) break;
+        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
+    }
+    flush_counters(p, t, ct);
 }
 
 /* ---- the plain kernel: one source per cooperative launch over the grid ---- */
@@ -837,8 +926,8 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     __shared__ Shared sh;
     Slot S;
     slot_layout(p.nmax, p.scratch, &S);
-    const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl)};
-    if (threadIdx.x == 0) sh.te = 0;
+    const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl), &sh.loc};
+    if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;
     View V;
     V.n = p.n;
@@ -848,9 +937,9 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];
     const uint64_t t0 = wall_clock64();
-    phase_distances(V, S, sh, t, src, p.delta, 0, p.err, ct);
+    phase_distances(V, S, t, src, p.delta, 0, p.err, ct);
     const uint64_t t1 = wall_clock64();
-    const uint32_t nlev = phase_sigma(V, S, sh, t, src, 0, p.err, ct);
+    const uint32_t nlev = phase_sigma(V, S, t, src, 0, p.err, ct);
     const uint64_t t2 = wall_clock64();
     phase_dependencies<MODE, 0, true>(p, V, S, t, task, 0, src, nlev);
     if (t.leader()) {
@@ -858,13 +947,16 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
         ct.c[STAT_T_SIGMA] += t2 - t1;
         ct.c[STAT_T_DEP] += wall_clock64() - t2;
     }
-    flush_counters(p, sh, ct);
+    flush_counters(p, t, ct);
 }
 
 template <int MODE, int PREC, bool ORDERED>
 static hipError_t launch_one(const BrandesParams &p, uint32_t slots, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_brandes<MODE, PREC, ORDERED>), dim3(slots), dim3(WG), 0, s, p);
+    if (wave_team(p.nmax))
+        hipLaunchKernelGGL((k_brandes_wave<MODE, PREC, ORDERED>), dim3((slots + WAVES - 1) / WAVES), dim3(WG), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_brandes<MODE, PREC, ORDERED>), dim3(slots), dim3(WG), 0, s, p);
     return hipGetLastError();
 }
 

@@ -133,13 +133,18 @@ uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
 int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out)
 {
     const uint64_t sb = fbcdev::slot_bytes(nmax);
-    uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)(ctx->cu_count * 8);
+    /* 8 workgroups per CU; a wavefront-per-source workgroup holds 4 slots */
+    const uint32_t per_cu = fbcdev::wave_team(nmax) ? 32u : 8u;
+    uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
     /* whole-graph slots (tens of MB each) stop at ~2 per CU: beyond that the
      * slots' working sets only contend for L2 and MALL */
     const size_t budget = std::min<size_t>(ctx->total_mem / 3, (size_t)24 << 30);
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
+    /* the wavefront kernel's workgroups address WAVES slots each: hold whole
+     * workgroups' worth */
+    if (fbcdev::wave_team(nmax)) slots = (slots + fbcdev::WAVES - 1) / fbcdev::WAVES * fbcdev::WAVES;
     const size_t need = (size_t)slots * sb;
     if (need > ctx->scratch_bytes) {
         dfree(ctx->scratch);
@@ -178,6 +183,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     if (rc) return rc;
     p.scratch = ctx->scratch;
     p.slot_bytes = fbcdev::slot_bytes(p.nmax);
+    p.nslots = slots;
     p.delta = auto_delta(ctx, g);
     p.sort_cap = ctx->sort_cap;
     p.queue_head = ctx->d_queue;

@@ -68,6 +68,7 @@ struct BrandesParams {
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
     uint32_t *grid_ctl;       /* plain kernel: traversal counters shared by the grid (16 words) */
     uint32_t sort_cap;        /* batched kernel: sort near queues / levels of up to this many vertices (0: off) */
+    uint32_t nslots;          /* slots the scratch holds (the wavefront kernel checks its slot against it) */
 };
 
 /* Largest near queue or level the batched kernel sorts (LDS words). */
@@ -87,6 +88,12 @@ enum Stat : int {
     STAT_T_DEP = 9,       /* time in the dependency phase */
     STAT_COUNT = 10
 };
+
+/* Views of at most this many vertices run one source per wavefront (their
+ * frontiers are a few dozen vertices wide); larger ones one per workgroup. */
+constexpr uint32_t WAVE_TEAM_MAX_N = 16384;
+constexpr int WAVES = WG / 64;             /* slots of a wavefront-per-source workgroup */
+inline bool wave_team(uint32_t nmax) { return nmax <= WAVE_TEAM_MAX_N; }
 
 /* Bytes of one slot for views of at most nmax vertices. */
 uint64_t slot_bytes(uint32_t nmax);
