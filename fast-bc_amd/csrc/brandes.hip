@@ -34,8 +34,9 @@ namespace fbcdev {
 
 constexpr int HEAVY_DEG = 96;   /* out-degree handled by the whole workgroup */
 constexpr int HEAVY_CAP = 64;
-constexpr int SUCC_REG = 8;     /* successors kept in registers by the ordered fold */
-constexpr int RB = 8;           /* arcs of one vertex whose loads and atomics are issued together */
+/* Per team: RB arcs of one vertex have their loads and atomics issued
+ * together; the ordered fold keeps up to SUCC successors in registers.  The
+ * wavefront team takes smaller blocks to stay at 4 wavefronts per SIMD. */
 
 /* ---- memory helpers ------------------------------------------------------ */
 
@@ -275,6 +276,7 @@ __device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
 /* One workgroup per source (batched kernel). */
 struct TeamWG {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
+    static constexpr int RB = 8, SUCC = 8;
     Ctl *ctl;
     Local *loc;
     uint32_t *sortbuf;
@@ -303,6 +305,7 @@ __device__ __forceinline__ void wave_sync()
  * part of one workgroup). */
 struct TeamWave {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
+    static constexpr int RB = 4, SUCC = 4;
     Ctl *ctl;
     Local *loc;
     __device__ uint32_t rank() const { return threadIdx.x & 63; }
@@ -321,6 +324,7 @@ struct TeamWave {
  * resident, so the grid barrier always completes). */
 struct TeamGrid {
     static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
+    static constexpr int RB = 8, SUCC = 8;
     Ctl *ctl;
     Local *loc;
     __device__ uint32_t rank() const { return blockIdx.x * WG + threadIdx.x; }
@@ -379,7 +383,7 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
                                              uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far,
                                              uint32_t *far_cnt, uint32_t *err, Counters &ct)
 {
-    constexpr int SC = T::SC;
+    constexpr int SC = T::SC, RB = T::RB;
     ct.c[STAT_RELAX] += e1 - e0;
     for (uint32_t eb = e0; eb < e1; eb += RB) {
         uint32_t to[RB];
@@ -548,7 +552,7 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
                                            uint32_t e0, uint32_t e1, uint32_t step, uint32_t *tail,
                                            uint32_t *err)
 {
-    constexpr int SC = T::SC;
+    constexpr int SC = T::SC, RB = T::RB;
     for (uint32_t eb = e0; eb < e1; eb += RB * step) {
         uint2 a[RB];
 #pragma unroll
@@ -580,7 +584,7 @@ template <class T>
 __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
-    constexpr int SC = T::SC;
+    constexpr int SC = T::SC, RB = T::RB;
     const uint32_t r0 = t.rank(), R = t.size();
     Ctl &c = *t.ctl;
     unsigned long long te = 0;
@@ -695,7 +699,7 @@ struct Acc<1> {
 /* Dependency of v: fold over DAG successors.  'ordered' sums them in
  * descending (d(w), w).  Distances, sigma and the successors' deltas are
  * final here, so plain loads see them after the level barrier. */
-template <int PREC, bool ORDERED>
+template <int PREC, bool ORDERED, int RB, int SUCC_REG>
 __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v)
 {
     Acc<PREC> acc;
@@ -734,10 +738,8 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
                 if ((uint64_t)dv + a[j].y == dw) key[j] = ((uint64_t)dw << 32) | a[j].x;
             }
         }
-        ulonglong2 sw[SUCC_REG];
-#pragma unroll
-        for (int j = 0; j < SUCC_REG; ++j) sw[j] = key[j] ? sd_pair(S, a[j].x) : make_ulonglong2(0, 0);
-        /* keys are distinct and nonzero for successors (dw >= 1) */
+        /* keys are distinct and nonzero for successors (dw >= 1); the low
+         * half of the best key is the successor's id */
 #pragma unroll
         for (int r = 0; r < SUCC_REG; ++r) {
             uint64_t best = 0;
@@ -746,10 +748,10 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
             for (int j = 0; j < SUCC_REG; ++j)
                 if (key[j] > best) { best = key[j]; bj = j; }
             if (bj < 0) break;
-            ulonglong2 b = make_ulonglong2(0, 0);
 #pragma unroll
             for (int j = 0; j < SUCC_REG; ++j)
-                if (j == bj) { key[j] = 0; b = sw[j]; }
+                if (j == bj) key[j] = 0;
+            const ulonglong2 b = sd_pair(S, (uint32_t)best);
             acc.add(sv_int, sv, b.x, b.y);
         }
         return acc;
@@ -785,7 +787,7 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
         for (uint32_t i = i0 + r0; i < i1; i += R) {
             const uint32_t v = S.qa[i];
             if (v == src) continue;
-            const Acc<PREC> acc = fold_successors<PREC, ORDERED>(V, S, v);
+            const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC>(V, S, v);
             S.sd[2 * (uint64_t)v + 1] = acc.bits();
             const double dv = acc.value();
             if (MODE == MODE_DEP) {
