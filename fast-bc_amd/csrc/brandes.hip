@@ -1058,6 +1058,19 @@ __global__ void k_pivot_subtract(uint64_t nv, const uint32_t *v_gid, const uint3
     bc[g] = b;
 }
 
+__global__ void k_gather_add(uint32_t n, const uint32_t *perm, const double *src, double *dst)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) dst[v] += src[perm[v]];
+}
+
+hipError_t launch_gather_add(uint32_t n, const uint32_t *perm, const double *src, double *dst, hipStream_t stream)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_add, dim3((n + 255) / 256), dim3(256), 0, stream, n, perm, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_pivot_subtract(uint64_t nv, const uint32_t *v_gid, const uint32_t *v_cluster,
                                  const uint32_t *cpiv_ptr, const uint32_t *piv_w,
                                  const double *intra, double *bc, hipStream_t stream)

@@ -27,6 +27,7 @@ struct fbc_ctx {
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
+    uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
     int cu_count = 0;
     size_t total_mem = 0;
     uint8_t *scratch = nullptr;
@@ -48,6 +49,14 @@ struct fbc_graph {
     std::vector<uint32_t> rp, col, w, rrp, rsrc, rw;   /* host copies */
     uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
     uint32_t *d_rp = nullptr, *d_rrp = nullptr;
+    /* Locality-relabelled copy for the modes whose sums are order-free
+     * (pivot phase, exact): vertex ids regrouped so that each 16-id block is a
+     * BFS-grown blob of neighbouring vertices; built on first use. */
+    bool blocked = false;
+    std::vector<uint32_t> perm;                 /* original id -> blocked id */
+    uint32_t *d_perm = nullptr;
+    uint2 *d_barcs = nullptr, *d_brarcs = nullptr;
+    uint32_t *d_brp = nullptr, *d_brrp = nullptr;
 };
 
 struct fbc_clusters {
@@ -293,6 +302,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     if (!std::strcmp(name, "delta")) ctx->delta_override = v;
     else if (!std::strcmp(name, "slots")) ctx->slots_override = v;
     else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
+    else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
     return 0;
 }
@@ -372,7 +382,119 @@ void fbc_graph_free(fbc_graph *g)
     dfree(g->d_rarcs);
     dfree(g->d_rp);
     dfree(g->d_rrp);
+    dfree(g->d_perm);
+    dfree(g->d_barcs);
+    dfree(g->d_brarcs);
+    dfree(g->d_brp);
+    dfree(g->d_brrp);
     delete g;
+}
+
+/* Blocked vertex order, hierarchical: the graph is cut into BFS-grown blobs
+ * of 4096 vertices (seeds in ascending id), each blob into blobs of 256, each
+ * of those into blobs of 16, and vertices are numbered blob by blob.  A
+ * 16-vertex blob's per-vertex state then shares cache lines, and blobs that
+ * are close in the graph are close in memory at every scale (the way a
+ * space-filling curve orders a grid). */
+static void blocked_order(const fbc_graph *g, std::vector<uint32_t> &perm)
+{
+    static const uint32_t SIZES[] = {4096, 256, 16};
+    const uint32_t n = g->n;
+    std::vector<uint32_t> region(n, 0), stamp(n, 0);   /* region id of each vertex at this level */
+    std::vector<uint32_t> order(n), next_order;
+    for (uint32_t v = 0; v < n; ++v) order[v] = v;
+    uint32_t nreg = 1, tick = 0;
+    std::vector<uint32_t> q;
+    for (uint32_t size : SIZES) {
+        /* split every region (a run of 'order') into BFS blobs of 'size' */
+        next_order.clear();
+        next_order.reserve(n);
+        std::vector<uint32_t> new_region(n);
+        uint32_t new_nreg = 0;
+        ++tick;   /* stamp == tick: already placed at this level */
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t s = order[i];
+            if (stamp[s] == tick) continue;
+            const uint32_t r = region[s];
+            q.clear();
+            q.push_back(s);
+            stamp[s] = tick;
+            for (size_t h = 0; h < q.size() && q.size() < size; ++h) {
+                const uint32_t u = q[h];
+                for (int dir = 0; dir < 2 && q.size() < size; ++dir) {
+                    const std::vector<uint32_t> &ptr = dir ? g->rrp : g->rp;
+                    const std::vector<uint32_t> &nb = dir ? g->rsrc : g->col;
+                    for (uint32_t e = ptr[u]; e < ptr[u + 1] && q.size() < size; ++e) {
+                        const uint32_t w = nb[e];
+                        if (stamp[w] == tick || region[w] != r) continue;
+                        stamp[w] = tick;
+                        q.push_back(w);
+                    }
+                }
+            }
+            for (uint32_t w : q) {
+                new_region[w] = new_nreg;
+                next_order.push_back(w);
+            }
+            ++new_nreg;
+        }
+        order.swap(next_order);
+        region.swap(new_region);
+        nreg = new_nreg;
+    }
+    (void)nreg;
+    perm.resize(n);
+    for (uint32_t i = 0; i < n; ++i) perm[order[i]] = i;
+}
+
+/* Build and upload the relabelled CSRs (forward and reverse, rows in the new
+ * order, arcs by ascending new target / source). */
+static int ensure_blocked(fbc_ctx *ctx, fbc_graph *g)
+{
+    if (g->blocked) return 0;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    try {
+        blocked_order(g, g->perm);
+        std::vector<uint32_t> inv(n);
+        for (uint32_t v = 0; v < n; ++v) inv[g->perm[v]] = v;
+        auto build = [&](const std::vector<uint32_t> &ptr, const std::vector<uint32_t> &nb,
+                         const std::vector<uint32_t> &wt, std::vector<uint32_t> &bp, std::vector<uint2> &arcs) {
+            bp.assign((size_t)n + 1, 0);
+            arcs.resize(m);
+            for (uint32_t nu = 0; nu < n; ++nu) bp[nu + 1] = bp[nu] + (ptr[inv[nu] + 1] - ptr[inv[nu]]);
+            for (uint32_t nu = 0; nu < n; ++nu) {
+                const uint32_t u = inv[nu];
+                uint2 *row = &arcs[bp[nu]];
+                for (uint32_t e = ptr[u]; e < ptr[u + 1]; ++e) row[e - ptr[u]] = make_uint2(g->perm[nb[e]], wt[e]);
+                std::sort(row, row + (ptr[u + 1] - ptr[u]), [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
+            }
+        };
+        std::vector<uint32_t> brp, brrp;
+        std::vector<uint2> barcs, brarcs;
+        build(g->rp, g->col, g->w, brp, barcs);
+        build(g->rrp, g->rsrc, g->rw, brrp, brarcs);
+        hipError_t e;
+        if ((e = upload(&g->d_barcs, barcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_brarcs, brarcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_brp, brp, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_brrp, brrp, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_perm, g->perm, ctx->stream)) != hipSuccess)
+            return hip_fail(ctx, e, "relabelled graph upload");
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, FBC_E_OOM, "host allocation");
+    }
+    g->blocked = true;
+    return 0;
+}
+
+/* Point p at the relabelled graph. */
+static void use_blocked(BrandesParams &p, const fbc_graph *g)
+{
+    p.arcs = g->d_barcs;
+    p.rp = g->d_brp;
+    p.rarcs = g->d_brarcs;
+    p.rrp = g->d_brrp;
 }
 
 /* Output-row modes (sssp / dep): sources in chunks that fit a device budget. */
@@ -462,8 +584,13 @@ int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc)
 {
     if (!ctx || !g || !bc) return fail(ctx, FBC_E_ARG, "null argument");
     if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    const bool blocked = ctx->relabel != 0;
+    if (blocked) {
+        const int rc = ensure_blocked(ctx, const_cast<fbc_graph *>(g));
+        if (rc) return rc;
+    }
     std::vector<uint32_t> src(g->n);
-    std::iota(src.begin(), src.end(), 0u);
+    std::iota(src.begin(), src.end(), 0u);   /* every vertex (in the graph's own numbering) */
     uint32_t *d_src = nullptr;
     double *d_bc = nullptr;
     hipError_t e;
@@ -475,12 +602,20 @@ int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc)
     }
     BrandesParams p;
     base_params(p, g);
+    if (blocked) use_blocked(p, g);
     p.ntasks = g->n;
     p.task_src = d_src;
     p.bc = d_bc;
     int rc = run(ctx, g, fbcdev::MODE_EXACT, prec, false, p);
-    if (!rc && (e = hipMemcpy(bc, d_bc, (size_t)g->n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+    if (!rc && blocked) {
+        std::vector<double> nb(g->n);
+        if ((e = hipMemcpy(nb.data(), d_bc, (size_t)g->n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = hip_fail(ctx, e, "result download");
+        else
+            for (uint32_t v = 0; v < g->n; ++v) bc[v] = nb[g->perm[v]];
+    } else if (!rc && (e = hipMemcpy(bc, d_bc, (size_t)g->n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess) {
         rc = hip_fail(ctx, e, "result download");
+    }
     dfree(d_src);
     dfree(d_bc);
     return rc;
@@ -760,8 +895,16 @@ static int pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *piv
         std::vector<uint32_t> fill(cptr.begin(), cptr.end() - 1);
         for (uint32_t i = 0; i < npiv; ++i) pw[fill[pivot_cluster[i]]++] = weight[i];
     }
+    /* the batched form runs on the relabelled graph and gathers its sums back */
+    const bool blocked = !plain && ctx->relabel != 0;
+    if (blocked) {
+        const int rc = ensure_blocked(ctx, const_cast<fbc_graph *>(g));
+        if (rc) return rc;
+    }
     std::vector<uint32_t> tsrc(pivot, pivot + npiv), taux(weight, weight + npiv);
-    double *d_bc = nullptr, *d_intra = nullptr;
+    if (blocked)
+        for (uint32_t &x : tsrc) x = g->perm[x];
+    double *d_bc = nullptr, *d_intra = nullptr, *d_bcb = nullptr;
     uint32_t *d_tsrc = nullptr, *d_taux = nullptr, *d_cptr = nullptr, *d_pw = nullptr;
     hipError_t e;
     int rc = 0;
@@ -771,15 +914,22 @@ static int pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *piv
         (e = hipMemcpy(d_bc, bc, (size_t)g->n * 8, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(d_intra, intra_bc, (size_t)g->n * 8, hipMemcpyHostToDevice)) != hipSuccess)
         rc = hip_fail(ctx, e, "pivot setup");
+    if (!rc && blocked &&
+        ((e = dalloc(&d_bcb, g->n)) != hipSuccess ||
+         (e = hipMemsetAsync(d_bcb, 0, (size_t)g->n * 8, ctx->stream)) != hipSuccess))
+        rc = hip_fail(ctx, e, "pivot setup");
     if (!rc) {
         BrandesParams p;
         base_params(p, g);
+        if (blocked) use_blocked(p, g);
         p.ntasks = npiv;
         p.task_src = d_tsrc;
         p.task_aux = d_taux;
-        p.bc = d_bc;
+        p.bc = blocked ? d_bcb : d_bc;
         rc = run(ctx, g, fbcdev::MODE_PIVOT, prec, false, p, plain);
     }
+    if (!rc && blocked && (e = fbcdev::launch_gather_add(g->n, g->d_perm, d_bcb, d_bc, ctx->stream)) != hipSuccess)
+        rc = hip_fail(ctx, e, "pivot gather");
     if (!rc) {
         e = fbcdev::launch_pivot_subtract(cl->gid.size(), cl->d_gid, cl->d_vcl, d_cptr, d_pw, d_intra, d_bc,
                                           ctx->stream);
@@ -788,6 +938,7 @@ static int pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *piv
         if (e != hipSuccess) rc = hip_fail(ctx, e, "pivot finish");
     }
     dfree(d_bc);
+    dfree(d_bcb);
     dfree(d_intra);
     dfree(d_tsrc);
     dfree(d_taux);

@@ -95,6 +95,10 @@ constexpr uint32_t WAVE_TEAM_MAX_N = 16384;
 constexpr int WAVES = WG / 64;             /* slots of a wavefront-per-source workgroup */
 inline bool wave_team(uint32_t nmax) { return nmax <= WAVE_TEAM_MAX_N; }
 
+/* dst[v] += src[perm[v]] for v < n (sums of a relabelled run back to the
+ * original numbering). */
+hipError_t launch_gather_add(uint32_t n, const uint32_t *perm, const double *src, double *dst, hipStream_t stream);
+
 /* Bytes of one slot for views of at most nmax vertices. */
 uint64_t slot_bytes(uint32_t nmax);
 

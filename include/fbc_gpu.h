@@ -71,8 +71,10 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
 
 /* Named tuning option (0 = automatic / off): "delta", "slots" (as above),
  * "frontier_sort" (the batched kernel sorts near queues and DAG levels of at
- * most this many vertices, up to 1024, before expanding them).  No
- * counterpart in the reference. */
+ * most this many vertices, up to 1024, before expanding them), "relabel"
+ * (1, the default: the batched pivot phase and fbc_exact_bc run on a copy of
+ * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
+ * locality; 0: on the graph as given).  No counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
