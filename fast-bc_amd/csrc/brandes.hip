@@ -867,7 +867,9 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
             const uint32_t b = p.cl_blocal[b0 + j];
             const uint32_t d = dist_final(S, b);
             p.vi_len[row + j] = d == DINF ? 0 : d;
-            p.vi_cnt[row + j] = S.sd[2 * (uint64_t)b];
+            /* the reference keeps (int)sigma: INT_MIN beyond the int range */
+            const uint64_t sg = S.sd[2 * (uint64_t)b];
+            p.vi_cnt[row + j] = sg > 0x7FFFFFFFull ? (int32_t)0x80000000u : (int32_t)sg;
         }
     }
     phase_dependencies<MODE, PREC, ORDERED>(p, V, S, t, task, c, src, nlev);

@@ -28,6 +28,7 @@ struct fbc_ctx {
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
+    uint64_t plain_fallbacks = 0;     /* plain-form runs finished on the batched kernel */
     int cu_count = 0;
     size_t total_mem = 0;
     uint8_t *scratch = nullptr;
@@ -210,8 +211,32 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
         const uint32_t want = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count;
         if (blocks > want) blocks = want;
         p.grid_ctl = ctx->d_grid_ctl;
-        for (uint32_t t = 0; t < p.ntasks; ++t)
-            HIP_TRY(ctx, fbcdev::launch_plain(mode, p, t, blocks, ctx->stream));
+        uint32_t t = 0;
+        for (; t < p.ntasks; ++t)
+            if (fbcdev::launch_plain(mode, p, t, blocks, ctx->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                break;
+            }
+        if (t < p.ntasks) {
+            /* The runtime refused a cooperative launch (another process on
+             * the device holds its cooperative-launch resources).  The
+             * remaining sources run on the batched kernel, which computes the
+             * same values (FP64, reference successor order). */
+            ++ctx->plain_fallbacks;
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   /* queued plain launches use the scratch */
+            BrandesParams q = p;
+            q.task_src = p.task_src + t;
+            q.task_aux = p.task_aux ? p.task_aux + t : nullptr;
+            q.ntasks = p.ntasks - t;
+            if (mode == fbcdev::MODE_DEP) q.out_dep = p.out_dep + (uint64_t)t * p.n;
+            uint32_t qslots = 0;
+            rc = ensure_scratch(ctx, q.nmax, q.ntasks, &qslots);
+            if (rc) return rc;
+            q.scratch = ctx->scratch;
+            q.nslots = qslots;
+            HIP_TRY(ctx, hipMemsetAsync(ctx->d_queue, 0, sizeof(uint32_t), ctx->stream));
+            HIP_TRY(ctx, fbcdev::launch_brandes(mode, 0, true, q, qslots, ctx->stream));
+        }
     } else {
         HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, p, slots, ctx->stream));
     }
@@ -760,20 +785,22 @@ void fbc_clusters_free(fbc_clusters *cl)
     delete cl;
 }
 
-int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uint64_t *vi_len,
-                     uint64_t *vi_cnt)
+int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uint32_t *vi_len,
+                     int32_t *vi_cnt)
 {
     if (!ctx || !cl || !intra_bc || (cl->vi_total && (!vi_len || !vi_cnt)))
         return fail(ctx, FBC_E_ARG, "null argument");
     const fbc_graph *g = cl->g;
     double *d_intra = nullptr, *d_dmat = nullptr;
-    uint64_t *d_vil = nullptr, *d_vic = nullptr;
+    uint32_t *d_vil = nullptr;
+    int32_t *d_vic = nullptr;
     uint32_t *d_tsrc = nullptr, *d_taux = nullptr;
     hipError_t e;
     int rc = 0;
     /* dependency matrices (n_c x n_c doubles): all bordered clusters at once
-     * when they fit in 8 GiB, else chunks; never less than the largest one */
-    const size_t dbudget = ((size_t)8 << 30) / sizeof(double);
+     * when they fit in a quarter of device memory (at most 64 GiB), else
+     * chunks; never less than the largest one */
+    const size_t dbudget = std::min<size_t>(ctx->total_mem / 4, (size_t)64 << 30) / sizeof(double);
     size_t dcap = 0, dall = 0;
     for (uint32_t c = 0; c < cl->ncl; ++c)
         if (cl->bptr[c + 1] > cl->bptr[c]) {
@@ -862,8 +889,8 @@ int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uin
     }
     if (!rc) {
         if ((e = hipMemcpy(intra_bc, d_intra, (size_t)g->n * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
-            (cl->vi_total && (e = hipMemcpy(vi_len, d_vil, cl->vi_total * 8, hipMemcpyDeviceToHost)) != hipSuccess) ||
-            (cl->vi_total && (e = hipMemcpy(vi_cnt, d_vic, cl->vi_total * 8, hipMemcpyDeviceToHost)) != hipSuccess))
+            (cl->vi_total && (e = hipMemcpy(vi_len, d_vil, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess) ||
+            (cl->vi_total && (e = hipMemcpy(vi_cnt, d_vic, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess))
             rc = hip_fail(ctx, e, "result download");
     }
     dfree(d_intra);

@@ -61,8 +61,8 @@ struct BrandesParams {
     uint64_t *out_dist;
     uint64_t *out_sigma;
     double *dmat;
-    uint64_t *vi_len;
-    uint64_t *vi_cnt;
+    uint32_t *vi_len;
+    int32_t *vi_cnt;
     uint32_t *err;
     unsigned long long *te;   /* traversed-edge counter */
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */

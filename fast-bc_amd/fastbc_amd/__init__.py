@@ -49,6 +49,7 @@ _lib = None
 _gen = None
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
@@ -79,7 +80,7 @@ def lib():
         L.fbc_clusters_edges.argtypes = [_vp, ctypes.c_uint32]
         L.fbc_clusters_edges.restype = ctypes.c_int64
         L.fbc_clusters_free.argtypes = [_vp]
-        L.fbc_cluster_eval.argtypes = [_vp, _vp, _f64p, _u64p, _u64p]
+        L.fbc_cluster_eval.argtypes = [_vp, _vp, _f64p, _u32p, _i32p]
         L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
                                       ctypes.c_int, _f64p]
         L.fbc_pivot_phase_plain.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p]
@@ -292,8 +293,8 @@ class Clusters:
         n = self.graph.n
         bc = np.zeros(n) if intra_bc is None else np.ascontiguousarray(intra_bc, np.float64).copy()
         tot = max(int(self.vi_off[-1]), 1)
-        L = np.zeros(tot, np.uint64)
-        C = np.zeros(tot, np.uint64)
+        L = np.zeros(tot, np.uint32)
+        C = np.zeros(tot, np.int32)
         self.graph.ctx.check(lib().fbc_cluster_eval(self.graph.ctx.h, self.h, bc, L, C))
         return bc, L, C
 

@@ -57,10 +57,6 @@ void finish_graph(Graph &g);
 std::vector<std::vector<int>> louvain(const Graph &g, const std::vector<uint64_t> &seeds,
                                       double precision);
 
-/* (int) of a double as x86-64 does it: values outside int range become INT_MIN.
- * The reference stores (int)sigma in VertexInfo (DijkstraClusterEvaluator.h:173). */
-int vi_count_from_sigma(uint64_t sigma);
-
 /* VertexInfoPivotSelector::selectPivots (VertexInfoPivotSelector.h:29-127) on
  * the rows of one cluster: len is normalized in place (VertexInfo.h:268-278);
  * rows with equal (count, length) vectors form a class in first-appearance

@@ -36,7 +36,8 @@ public:
         fbc_clusters *cl = nullptr;
         dev.check(fbc_clusters_create(dev.ctx(), dg, 1, ptr.data(), verts.data(), &cl, bptr));
         const uint32_t nb = bptr[1];
-        std::vector<uint64_t> len((size_t)vs.size() * nb), cnt((size_t)vs.size() * nb);
+        std::vector<uint32_t> len((size_t)vs.size() * nb);
+        std::vector<int32_t> cnt((size_t)vs.size() * nb);
         /* the cluster's sum starts from zero and is then added to clusterBC,
          * as the reference's private reduction copy is (DijkstraClusterEvaluator.h:66) */
         std::vector<double> bc(clusterBC.size(), 0.0);
@@ -48,7 +49,7 @@ public:
             auto vi = std::make_shared<VertexInfo<V, W>>((int)nb);
             for (uint32_t j = 0; j < nb; ++j) {
                 vi->setBorderSPLength((int)j, (W)len[r * nb + j]);
-                vi->setBorderSPCount((int)j, (V)fbchost::vi_count_from_sigma(cnt[r * nb + j]));
+                vi->setBorderSPCount((int)j, (V)cnt[r * nb + j]);
             }
             globalVI[vs[r]] = vi;
         }

@@ -23,13 +23,6 @@
 
 namespace fbchost {
 
-int vi_count_from_sigma(uint64_t sigma)
-{
-    /* the reference converts its FP64 sigma to int: cvttsd2si gives INT_MIN
-     * for values beyond the int range */
-    return sigma > (uint64_t)INT_MAX ? INT_MIN : (int)sigma;
-}
-
 namespace {
 
 uint64_t mix(uint64_t h, uint64_t x)

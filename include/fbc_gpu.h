@@ -130,11 +130,12 @@ void fbc_clusters_free(fbc_clusters *cl);
  *                    vertex of the set) and border j the entry sits at
  *                    vi_off[c] + r * B_c + j, vi_off[c] = sum over c' < c of
  *                    n_c' * B_c'.  len = path length to the border (0 when
- *                    unreached), cnt = sigma (uint64; the reference keeps
- *                    (int)sigma).
+ *                    unreached), cnt = (int)sigma as the reference converts
+ *                    its FP64 count (INT_MIN beyond the int range,
+ *                    DijkstraClusterEvaluator.h:166-177).
  * Borderless clusters are skipped (FASTBC_BRANDES_CLUSTERED_IGNORE_UNCONNECTED). */
 int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc,
-                     uint64_t *vi_len, uint64_t *vi_cnt);
+                     uint32_t *vi_len, int32_t *vi_cnt);
 
 /* The pivot loop of ClusteredBrandeBC::computeBC (ClusteredBrandesBC.h:141-166):
  *   bc[v] += w_p * dep_p(v)                for every v,
@@ -151,7 +152,10 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
  * pivot with FP64 sums in the reference's successor order.  Same result
  * contract as fbc_pivot_phase with FBC_PREC_F64; the baseline the batched
  * form is measured against.  fbc_set_tuning's 'slots' sets the workgroups of
- * the grid (0: one per CU). */
+ * the grid (0: one per CU).  If the runtime refuses cooperative launches
+ * (another process holds the device's cooperative-launch resources), this
+ * and fbc_single_source_dep finish on the batched kernel, with the same
+ * values. */
 int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
                           const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
                           const double *intra_bc, double *bc);
