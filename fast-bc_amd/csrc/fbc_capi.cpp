@@ -988,7 +988,7 @@ int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *
 }
 
 int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
-                     const uint32_t *k, const double *features, const int32_t *weights, double min_variance,
+                     const uint32_t *k, const int32_t *features, const int32_t *weights, double min_variance,
                      uint32_t max_iteration, uint32_t *centroids, int32_t *cweights)
 {
     if (!ctx || (nsets && (!npoints || !nfeat || !k || !features || !weights || !centroids || !cweights)))
@@ -1013,15 +1013,10 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
     for (uint32_t s = 0; s < nsets; ++s) {
         double mx = 0;
-        const double *X = features + xoff[s];
-        for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) {
-            const double x = X[i];
-            if (x != std::floor(x)) bad |= 1;
-            mx = std::max(mx, std::fabs(x));
-        }
+        const int32_t *X = features + xoff[s];
+        for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) mx = std::max(mx, std::fabs((double)X[i]));
         if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0) bad |= 2;
     }
-    if (bad & 1) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features must be integers");
     if (bad & 2) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
     /* groups of sets whose Gram matrices fit the budget together */
     const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30), 1);
@@ -1048,7 +1043,8 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     p.min_variance = min_variance;
     uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtiles = nullptr;
     uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr, *dgo = nullptr;
-    double *dX = nullptr, *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmsum = nullptr, *dG = nullptr,
+    int32_t *dX = nullptr;
+    double *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmsum = nullptr, *dG = nullptr,
            *dkv = nullptr;
     int32_t *dW = nullptr, *dcw = nullptr;
     uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
@@ -1175,7 +1171,8 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     }
     for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
     for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
-    for (auto *q : {dX, dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
+    dfree(dX);
+    for (auto *q : {dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
     dfree(dW);
     dfree(dcw);
     ctx->last_ms = ms_total;

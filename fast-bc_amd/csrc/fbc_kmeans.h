@@ -18,7 +18,7 @@ struct KmParams {
     const uint64_t *koff;               /* per-centroid arrays */
     const uint64_t *moff;               /* member-sum block (K * D doubles) */
     const uint64_t *goff;               /* Gram block of the set in G (P * P doubles) */
-    const double *X;                    /* features, row-major */
+    const int32_t *X;                   /* integer features, row-major */
     double *xnorm;                      /* squared norm of every row (exact; the Gram diagonal) */
     const double *G;                    /* Gram matrices X X^T of the active sets (exact) */
     const int32_t *weights;             /* per point */

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const ui
 {
     const uint32_t s = tiles[3 * blockIdx.x], ti = tiles[3 * blockIdx.x + 1], tj = tiles[3 * blockIdx.x + 2];
     const uint32_t P = p.P[s], D = p.D[s];
-    const double *X = p.X + p.xoff[s];
+    const int32_t *X = p.X + p.xoff[s];
     double *Gs = G + p.goff[s];
     __shared__ double sa[GD][GT + 1];
     __shared__ double sb[GD][GT + 1];
@@ -106,8 +106,8 @@ __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const ui
         for (int e = threadIdx.x; e < GD * GT; e += GWG) {
             const int dd = e % GD, rr = e / GD;
             const uint32_t f = d0 + dd, ra = r0 + rr, rb = c0 + rr;
-            sa[dd][rr] = (ra < P && f < D) ? X[(uint64_t)ra * D + f] : 0.0;
-            sb[dd][rr] = (rb < P && f < D) ? X[(uint64_t)rb * D + f] : 0.0;
+            sa[dd][rr] = (ra < P && f < D) ? (double)X[(uint64_t)ra * D + f] : 0.0;
+            sb[dd][rr] = (rb < P && f < D) ? (double)X[(uint64_t)rb * D + f] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(AWG) void k_km_msum(KmParams p, const uint32_t *til
     if (!p.active[c]) return;
     const uint32_t m0 = tiles[3 * blockIdx.x + 1], f = tiles[3 * blockIdx.x + 2] + threadIdx.x;
     const uint32_t P = p.P[c], D = p.D[c];
-    const double *X = p.X + p.xoff[c];
+    const int32_t *X = p.X + p.xoff[c];
     const uint32_t *asg = p.assign + p.poff[c];
     const uint32_t *members = p.members + p.poff[c];
     double *msum = p.msum + p.moff[c];
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(AWG) void k_km_msum(KmParams p, const uint32_t *til
             acc = 0;
             k = s_k[i];
         }
-        acc += X[(uint64_t)s_v[i] * D + f];
+        acc += (double)X[(uint64_t)s_v[i] * D + f];
     }
     unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
 }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(AWG) void k_km_dist(KmParams p, const uint32_t *til
     if (!p.active[c]) return;
     const uint32_t m0 = tile_m0[blockIdx.x];
     const uint32_t P = p.P[c], D = p.D[c];
-    const double *X = p.X + p.xoff[c];
+    const int32_t *X = p.X + p.xoff[c];
     const uint32_t *asg = p.assign + p.poff[c];
     const uint32_t *cnt = p.cnt + p.koff[c];
     const uint32_t *members = p.members + p.poff[c];
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(AWG) void k_km_dist(KmParams p, const uint32_t *til
         if (f < D) {
 #pragma unroll 4
             for (uint32_t mi = wave; mi < nm; mi += AWG / 64)
-                s_d[mi][lane] = msum[(uint64_t)s_k[mi] * D + f] / s_n[mi] - X[(uint64_t)s_v[mi] * D + f];
+                s_d[mi][lane] = msum[(uint64_t)s_k[mi] * D + f] / s_n[mi] - (double)X[(uint64_t)s_v[mi] * D + f];
         }
         __syncthreads();
         if (threadIdx.x < nm) {

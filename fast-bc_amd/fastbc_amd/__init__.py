@@ -87,7 +87,8 @@ def lib():
         L.fbc_set_option.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int64]
         L.fbc_last_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.fbc_last_counters.argtypes = [_vp, _u64p, ctypes.c_int]
-        L.fbc_kmeans_batch.argtypes = [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p, _f64p,
+        L.fbc_kmeans_batch.argtypes = [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p,
+                                       np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                        np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), ctypes.c_double,
                                        ctypes.c_uint32, _u32p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -155,11 +156,16 @@ class Context:
         return Graph(self, n, row_ptr, col, w)
 
     def kmeans_batch(self, sets, min_variance=0.0, max_iteration=100):
-        """PlusPlusKMeans on the GPU for a list of (k, features [P x D], weights)."""
+        """PlusPlusKMeans on the GPU for a list of (k, features [P x D], weights).
+
+        Features are integer VertexInfo rows (ValueError for other values)."""
         np_ = _u32([len(f) for _, f, _ in sets])
         nf = _u32([f.shape[1] for _, f, _ in sets])
         kk = _u32([k for k, _, _ in sets])
-        feat = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float64).reshape(-1) for _, f, _ in sets]))
+        flat = np.concatenate([np.asarray(f).reshape(-1) for _, f, _ in sets])
+        feat = np.ascontiguousarray(flat, np.int32)
+        if not np.array_equal(feat, flat):
+            raise ValueError("k-means features must be integers within the int32 range")
         w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for _, _, x in sets]), np.int32)
         oc = np.zeros(int(kk.sum()), np.uint32)
         ow = np.zeros(int(kk.sum()), np.int32)

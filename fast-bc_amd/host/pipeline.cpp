@@ -45,17 +45,26 @@ void kmeans_stage(Device &dev, uint32_t ncl, const std::vector<uint32_t> &bptr, 
                   PhaseTimes &t)
 {
     const double t0 = now();
-    std::vector<uint32_t> gsets, hsets;
+    /* a set goes to the GPU when its rows are integers small enough for exact
+     * FP64 distances (which also makes them fit int32) and k is supported */
+    std::vector<char> on_gpu(ncl, 0);
+#pragma omp parallel for schedule(dynamic, 16)
     for (uint32_t c = 0; c < ncl; ++c) {
         if (prow[c].empty()) continue;
         const uint32_t nb = bptr[c + 1] - bptr[c];
         double mx = 0;
-        for (double x : plen[c]) mx = std::max(mx, std::fabs(x));
+        bool integral = true;
+        for (double x : plen[c]) {
+            mx = std::max(mx, std::fabs(x));
+            integral &= x == std::floor(x);
+        }
         for (int x : pcnt[c]) mx = std::max(mx, std::fabs((double)x));
         const int kc = std::max((int)(prow[c].size() * kfrac), 1);
-        const bool exact_on_gpu = 2.0 * 2 * nb * mx * mx < 9007199254740992.0 && kc <= FBC_KMEANS_MAX_K;
-        (exact_on_gpu ? gsets : hsets).push_back(c);
+        on_gpu[c] = integral && 2.0 * 2 * nb * mx * mx < 9007199254740992.0 && kc <= FBC_KMEANS_MAX_K;
     }
+    std::vector<uint32_t> gsets, hsets;
+    for (uint32_t c = 0; c < ncl; ++c)
+        if (!prow[c].empty()) (on_gpu[c] ? gsets : hsets).push_back(c);
     std::vector<std::vector<int>> cent(ncl), cw(ncl);
     if (!gsets.empty()) {
         std::vector<uint32_t> np, nf, kk;
@@ -70,15 +79,15 @@ void kmeans_stage(Device &dev, uint32_t ncl, const std::vector<uint32_t> &bptr, 
             woff[i + 1] = woff[i] + P;
         }
         /* feature rows: length then count per border */
-        std::vector<double> feat(foff.back());
+        std::vector<int32_t> feat(foff.back());
         std::vector<int32_t> w(woff.back());
 #pragma omp parallel for schedule(dynamic, 1)
         for (size_t i = 0; i < gsets.size(); ++i) {
             const uint32_t c = gsets[i], P = np[i], nb = nf[i] / 2;
-            double *f = feat.data() + foff[i];
+            int32_t *f = feat.data() + foff[i];
             for (size_t e = 0; e < (size_t)P * nb; ++e) {
-                f[2 * e] = plen[c][e];
-                f[2 * e + 1] = (double)pcnt[c][e];
+                f[2 * e] = (int32_t)plen[c][e];
+                f[2 * e + 1] = pcnt[c][e];
             }
             std::copy(pw[c].begin(), pw[c].end(), w.begin() + woff[i]);
         }

@@ -165,16 +165,16 @@ int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *
  * KMeansPivotSelector calls it, brandes/KMeansPivotSelector.h:99-101).
  * Set s has npoints[s] rows of nfeat[s] features (row-major; rows of all sets
  * concatenated in 'features'), npoints[s] weights and k[s] centroids.
- * Features are VertexInfo rows (length then count per border) and must be
- * integers with 2 * nfeat * max|feature|^2 < 2^53, which makes every
- * point-to-point distance exact, and k[s] <= FBC_KMEANS_MAX_K
- * (FBC_E_UNSUPPORTED otherwise).  Outputs, k[s] entries per set,
+ * Features are VertexInfo rows (length then count per border), integers
+ * with 2 * nfeat * max|feature|^2 < 2^53, which makes every point-to-point
+ * distance exact, and k[s] <= FBC_KMEANS_MAX_K (FBC_E_UNSUPPORTED
+ * otherwise).  Outputs, k[s] entries per set,
  * concatenated: centroid point indices and their weights.  Each set's Gram
  * matrix (npoints^2 doubles) is held on the device while the set runs; sets
  * are processed in groups that fit a fraction of device memory. */
 #define FBC_KMEANS_MAX_K 12288
 int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
-                     const uint32_t *k, const double *features, const int32_t *weights,
+                     const uint32_t *k, const int32_t *features, const int32_t *weights,
                      double min_variance, uint32_t max_iteration, uint32_t *centroids,
                      int32_t *cweights);
 
