@@ -27,6 +27,7 @@ struct fbc_ctx {
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
+    uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
     uint64_t plain_fallbacks = 0;     /* plain-form runs finished on the batched kernel */
     int cu_count = 0;
@@ -150,7 +151,8 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     if (slots < 1) slots = 1;
     /* whole-graph slots (tens of MB each) stop at ~2 per CU: beyond that the
      * slots' working sets only contend for L2 and MALL */
-    const size_t budget = std::min<size_t>(ctx->total_mem / 3, (size_t)24 << 30);
+    const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20
+                                          : std::min<size_t>(ctx->total_mem / 3, (size_t)24 << 30);
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     /* the wavefront kernel's workgroups address WAVES slots each: hold whole
      * workgroups' worth */
@@ -328,6 +330,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "slots")) ctx->slots_override = v;
     else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
     else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
+    else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
     return 0;
 }

@@ -19,6 +19,14 @@
  * partition2graph is replaced by a stable sort with the same summation order;
  * weighted degrees, which the reference re-sums on every call with the same
  * order, are summed once per level.
+ *
+ * A pass is a chain of dependent random reads (the shuffled node, its links,
+ * its neighbours' communities, their sums), so each level packs a node's
+ * in-links then out-links into one run of (neighbour, weight) records shared
+ * by the instances, keeps a community's two sums side by side, and prefetches
+ * that chain a few nodes ahead along the shuffled order.  Aggregation reduces
+ * the communities in parallel (each one's sums keep their order) and adds the
+ * level's total in the reference's order afterwards.
  */
 #include "fbc_host.h"
 
@@ -47,6 +55,7 @@ struct LGraph {
         win.assign(nodes, 0.0);
         wout.assign(nodes, 0.0);
         self.assign(nodes, 0.0);
+#pragma omp parallel for schedule(static, 4096)
         for (unsigned v = 0; v < nodes; ++v) {
             double r = 0;
             for (uint64_t i = in_begin((int)v); i < in_end[v]; ++i) r += in_w.empty() ? 1.0 : in_w[i];
@@ -84,57 +93,102 @@ LGraph from_graph(const Graph &g)
     return L;
 }
 
+/* One level's links in the order the pass reads them: per node its in-links
+ * then its out-links (weights as the reference reads them, 1 for an
+ * unweighted graph), so a node's neighbourhood is one contiguous run. */
+struct Link {
+    int nb;
+    double w;
+};
+struct NodeRec {
+    uint64_t a0;          /* first link */
+    uint32_t nin, nout;
+    double wout, win;     /* weighted out / in degree */
+};
+struct CommTot {
+    double out, in;       /* woutctot, winctot */
+};
+
+/* The packed links of one level, shared by its instances. */
+struct LevelLinks {
+    std::vector<NodeRec> rec;
+    std::vector<Link> adj;
+
+    explicit LevelLinks(const LGraph &gr)
+    {
+        const int n = (int)gr.nodes;
+        rec.resize(n);
+        adj.resize(gr.in_nb.size() + gr.out_nb.size());
+        const bool win_unit = gr.in_w.empty(), wout_unit = gr.out_w.empty();
+#pragma omp parallel for schedule(static, 4096)
+        for (int i = 0; i < n; ++i) {
+            NodeRec &r = rec[i];
+            r.a0 = gr.in_begin(i) + gr.out_begin(i);
+            r.nin = (uint32_t)(gr.in_end[i] - gr.in_begin(i));
+            r.nout = (uint32_t)(gr.out_end[i] - gr.out_begin(i));
+            r.wout = gr.wout[i];
+            r.win = gr.win[i];
+            uint64_t a = r.a0;
+            for (uint64_t k = gr.in_begin(i); k < gr.in_end[i]; ++k) adj[a++] = {gr.in_nb[k], win_unit ? 1. : gr.in_w[k]};
+            for (uint64_t k = gr.out_begin(i); k < gr.out_end[i]; ++k)
+                adj[a++] = {gr.out_nb[k], wout_unit ? 1. : gr.out_w[k]};
+        }
+    }
+};
+
 struct Part {
     const LGraph *g;
     int size;
     double min_mod;
+    const std::vector<NodeRec> &rec;
+    const std::vector<Link> &adj;
     std::vector<double> nw;
     std::vector<unsigned> npos;
     unsigned nlast = 0;
     std::vector<int> n2c;
-    std::vector<double> woutc, winc, woutctot, winctot;
+    std::vector<double> woutc;
+    std::vector<CommTot> tot;
 
-    Part(const LGraph &gr, double minm) : g(&gr), size((int)gr.nodes), min_mod(minm)
+    Part(const LGraph &gr, const LevelLinks &ll, double minm)
+        : g(&gr), size((int)gr.nodes), min_mod(minm), rec(ll.rec), adj(ll.adj)
     {
         nw.assign(size, -1);
         npos.assign(size, 0);
         n2c.resize(size);
         woutc.resize(size);
-        winc.resize(size);
-        woutctot.resize(size);
-        winctot.resize(size);
+        tot.resize(size);
         for (int i = 0; i < size; ++i) {
             n2c[i] = i;
-            woutc[i] = winc[i] = gr.self[i];
-            woutctot[i] = gr.wout[i];
-            winctot[i] = gr.win[i];
+            woutc[i] = gr.self[i];
+            tot[i] = {gr.wout[i], gr.win[i]};
         }
     }
 
+    /* winc (the in-link twin of woutc) is kept by the reference but read by
+     * neither the gain nor the modularity, so it is not kept here. */
     void remove(int node)
     {
         woutc[node] = 0;
-        winc[node] = 0;
-        woutctot[n2c[node]] -= g->wout[node];
-        winctot[n2c[node]] -= g->win[node];
+        tot[n2c[node]].out -= rec[node].wout;
+        tot[n2c[node]].in -= rec[node].win;
         n2c[node] = -1;
     }
 
     void insert(int node, int comm)
     {
         n2c[node] = comm;
-        for (uint64_t i = g->out_begin(node); i < g->out_end[node]; ++i)
-            if (n2c[g->out_nb[i]] == comm) woutc[node] += g->out_w[i];
-        for (uint64_t i = g->in_begin(node); i < g->in_end[node]; ++i)
-            if (n2c[g->in_nb[i]] == comm) winc[node] += g->in_w[i];
-        woutctot[comm] += g->wout[node];
-        winctot[comm] += g->wout[node];
+        const NodeRec &r = rec[node];
+        const Link *out = adj.data() + r.a0 + r.nin;
+        for (uint32_t i = 0; i < r.nout; ++i)
+            if (n2c[out[i].nb] == comm) woutc[node] += out[i].w;
+        tot[comm].out += r.wout;
+        tot[comm].in += r.wout;
     }
 
     double gain(int node, int comm, double wic) const
     {
-        const double woutn = g->wout[node], winn = g->win[node];
-        const double wc_in = winctot[comm], wc_out = woutctot[comm];
+        const double woutn = rec[node].wout, winn = rec[node].win;
+        const double wc_in = tot[comm].in, wc_out = tot[comm].out;
         const double m = g->total_weight;
         return (wic / m - (woutn / m) * (wc_in / m) - (winn / m) * (wc_out / m));
     }
@@ -144,7 +198,7 @@ struct Part {
         double q = 0.;
         const double m = g->total_weight;
         for (int i = 0; i < size; ++i)
-            if (g->wout[i] > 0) q += woutc[i] / m - (g->wout[i] / m) * (winctot[n2c[i]] / m);
+            if (rec[i].wout > 0) q += woutc[i] / m - (rec[i].wout / m) * (tot[n2c[i]].in / m);
         return q;
     }
 
@@ -155,26 +209,52 @@ struct Part {
         npos[0] = (unsigned)n2c[node];
         nw[npos[0]] = 0;
         nlast = 1;
-        const bool win_unit = g->in_w.empty(), wout_unit = g->out_w.empty();
-        for (uint64_t i = g->in_begin(node); i < g->in_end[node]; ++i) {
-            const int nb = g->in_nb[i];
+        const NodeRec &r = rec[node];
+        const Link *l = adj.data() + r.a0;
+        const uint32_t nl = r.nin + r.nout;
+        for (uint32_t i = 0; i < nl; ++i) {
+            const int nb = l[i].nb;
             if (nb == node) continue;
             const unsigned c = (unsigned)n2c[nb];
             if (nw[c] == -1) {
                 nw[c] = 0.;
                 npos[nlast++] = c;
             }
-            nw[c] += win_unit ? 1. : g->in_w[i];
+            nw[c] += l[i].w;
         }
-        for (uint64_t i = g->out_begin(node); i < g->out_end[node]; ++i) {
-            const int nb = g->out_nb[i];
-            if (nb == node) continue;
-            const unsigned c = (unsigned)n2c[nb];
-            if (nw[c] == -1) {
-                nw[c] = 0.;
-                npos[nlast++] = c;
+    }
+
+    /* Prefetches for the node handled 'ahead' steps later: its record, then
+     * its links, then its neighbours' communities, then their sums.  A
+     * prefetch never changes a value; a community that changes before the
+     * node is reached only costs a miss. */
+    void prefetch(const std::vector<int> &order, int t) const
+    {
+        constexpr int A1 = 24, A2 = 12, A3 = 6, A4 = 3;
+        if (t + A1 < size) __builtin_prefetch(&rec[order[t + A1]]);
+        if (t + A2 < size) {
+            const int v = order[t + A2];
+            const NodeRec &r = rec[v];
+            __builtin_prefetch(&adj[r.a0]);
+            if (r.nin + r.nout > 4) __builtin_prefetch(&adj[r.a0 + r.nin + r.nout - 1]);
+            __builtin_prefetch(&n2c[v]);
+            __builtin_prefetch(&woutc[v]);
+        }
+        if (t + A3 < size) {
+            const NodeRec &r = rec[order[t + A3]];
+            const Link *l = adj.data() + r.a0;
+            for (uint32_t i = 0; i < r.nin + r.nout; ++i) __builtin_prefetch(&n2c[l[i].nb]);
+        }
+        if (t + A4 < size) {
+            const NodeRec &r = rec[order[t + A4]];
+            const Link *l = adj.data() + r.a0;
+            for (uint32_t i = 0; i < r.nin + r.nout; ++i) {
+                const int c = n2c[l[i].nb];
+                if (c >= 0) {
+                    __builtin_prefetch(&nw[c]);
+                    __builtin_prefetch(&tot[c]);
+                }
             }
-            nw[c] += wout_unit ? 1. : g->out_w[i];
         }
     }
 
@@ -190,6 +270,7 @@ struct Part {
             cur_mod = new_mod;
             moves = 0;
             for (int t = 0; t < size; ++t) {
+                prefetch(order, t);
                 const int node = order[t];
                 const int node_comm = n2c[node];
                 neigh_comm(node);
@@ -230,45 +311,86 @@ struct Part {
         }
         LGraph g2;
         g2.nodes = (unsigned)fin;
-        g2.total_weight = 0;
         g2.in_end.resize(fin);
         g2.out_end.resize(fin);
-        std::vector<std::pair<int, double>> inl, outl;
-        std::vector<size_t> idx;
-        auto emit = [&](std::vector<std::pair<int, double>> &lst, std::vector<int> &nb,
-                        std::vector<double> &wt, bool count_total) {
-            idx.resize(lst.size());
-            std::iota(idx.begin(), idx.end(), (size_t)0);
-            std::stable_sort(idx.begin(), idx.end(),
-                             [&](size_t a, size_t b) { return lst[a].first < lst[b].first; });
-            size_t k = 0;
-            while (k < idx.size()) {
-                const int key = lst[idx[k]].first;
-                double s = lst[idx[k]].second;
-                size_t j = k + 1;
-                while (j < idx.size() && lst[idx[j]].first == key) s += lst[idx[j++]].second;
-                if (count_total) g2.total_weight += s;
-                nb.push_back(key);
-                wt.push_back(s);
-                k = j;
-            }
-        };
-        const bool win_unit = g->in_w.empty(), wout_unit = g->out_w.empty();
+        /* each community's reduced lists go to a slot bounded by its members'
+         * link count; communities are independent, so they run in parallel */
+        std::vector<uint64_t> ibound((size_t)fin + 1, 0), obound((size_t)fin + 1, 0);
         for (int c = 0; c < fin; ++c) {
-            inl.clear();
-            outl.clear();
+            uint64_t bi = 0, bo = 0;
             for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
                 const int node = members[k];
-                for (uint64_t i = g->in_begin(node); i < g->in_end[node]; ++i)
-                    inl.push_back({renumber[n2c[g->in_nb[i]]], win_unit ? 1. : g->in_w[i]});
-                for (uint64_t i = g->out_begin(node); i < g->out_end[node]; ++i)
-                    outl.push_back({renumber[n2c[g->out_nb[i]]], wout_unit ? 1. : g->out_w[i]});
+                bi += g->in_end[node] - g->in_begin(node);
+                bo += g->out_end[node] - g->out_begin(node);
             }
-            emit(inl, g2.in_nb, g2.in_w, true);
-            emit(outl, g2.out_nb, g2.out_w, false);
-            g2.in_end[c] = g2.in_nb.size();
-            g2.out_end[c] = g2.out_nb.size();
+            ibound[c + 1] = ibound[c] + bi;
+            obound[c + 1] = obound[c] + bo;
         }
+        std::vector<std::pair<int, double>> ired(ibound[fin]), ored(obound[fin]);
+        std::vector<uint64_t> icount(fin), ocount(fin);
+        const bool win_unit = g->in_w.empty(), wout_unit = g->out_w.empty();
+        auto by_key = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
+        /* stable by key: each key's weights are summed in node order, then
+         * link order, as the reference's per-community map accumulates them */
+        auto reduce = [&](std::vector<std::pair<int, double>> &lst, std::pair<int, double> *out) {
+            std::stable_sort(lst.begin(), lst.end(), by_key);
+            uint64_t o = 0;
+            size_t k = 0;
+            while (k < lst.size()) {
+                const int key = lst[k].first;
+                double sum = lst[k].second;
+                size_t j2 = k + 1;
+                while (j2 < lst.size() && lst[j2].first == key) sum += lst[j2++].second;
+                out[o++] = {key, sum};
+                k = j2;
+            }
+            return o;
+        };
+#pragma omp parallel
+        {
+            std::vector<std::pair<int, double>> inl, outl;
+#pragma omp for schedule(dynamic, 256)
+            for (int c = 0; c < fin; ++c) {
+                inl.clear();
+                outl.clear();
+                for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+                    const int node = members[k];
+                    for (uint64_t i2 = g->in_begin(node); i2 < g->in_end[node]; ++i2)
+                        inl.push_back({renumber[n2c[g->in_nb[i2]]], win_unit ? 1. : g->in_w[i2]});
+                    for (uint64_t i2 = g->out_begin(node); i2 < g->out_end[node]; ++i2)
+                        outl.push_back({renumber[n2c[g->out_nb[i2]]], wout_unit ? 1. : g->out_w[i2]});
+                }
+                icount[c] = reduce(inl, ired.data() + ibound[c]);
+                ocount[c] = reduce(outl, ored.data() + obound[c]);
+            }
+        }
+        uint64_t ni = 0, no = 0;
+        for (int c = 0; c < fin; ++c) {
+            ni += icount[c];
+            no += ocount[c];
+            g2.in_end[c] = ni;
+            g2.out_end[c] = no;
+        }
+        g2.in_nb.resize(ni);
+        g2.in_w.resize(ni);
+        g2.out_nb.resize(no);
+        g2.out_w.resize(no);
+#pragma omp parallel for schedule(static, 1024)
+        for (int c = 0; c < fin; ++c) {
+            const uint64_t di = c ? g2.in_end[c - 1] : 0, dout = c ? g2.out_end[c - 1] : 0;
+            for (uint64_t k = 0; k < icount[c]; ++k) {
+                g2.in_nb[di + k] = ired[ibound[c] + k].first;
+                g2.in_w[di + k] = ired[ibound[c] + k].second;
+            }
+            for (uint64_t k = 0; k < ocount[c]; ++k) {
+                g2.out_nb[dout + k] = ored[obound[c] + k].first;
+                g2.out_w[dout + k] = ored[obound[c] + k].second;
+            }
+        }
+        /* the total is summed over the reduced in-links in (community, key)
+         * order, as they are emitted */
+        g2.total_weight = 0;
+        for (uint64_t k = 0; k < ni; ++k) g2.total_weight += g2.in_w[k];
         g2.derive();
         return g2;
     }
@@ -289,9 +411,10 @@ std::vector<std::vector<int>> louvain(const Graph &g, const std::vector<uint64_t
     std::iota(n2c.begin(), n2c.end(), 0);
     bool improvement;
     do {
+        const LevelLinks links(lg);
         std::vector<Part> parts;
         parts.reserve(P);
-        for (int i = 0; i < P; ++i) parts.emplace_back(lg, precision);
+        for (int i = 0; i < P; ++i) parts.emplace_back(lg, links, precision);
         std::vector<char> imp((size_t)P);
         std::vector<double> mods((size_t)P);
 #pragma omp parallel for schedule(static, 1)

@@ -74,7 +74,9 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * most this many vertices, up to 1024, before expanding them), "relabel"
  * (1, the default: the batched pivot phase and fbc_exact_bc run on a copy of
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
- * locality; 0: on the graph as given).  No counterpart in the reference. */
+ * locality; 0: on the graph as given), "scratch_mb" (device memory for the
+ * sources in flight, in MiB; automatic: a third of the device, at most
+ * 24 GiB).  No counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
