@@ -679,6 +679,7 @@ struct Acc<0> {
     }
     __device__ uint64_t bits() const { return (uint64_t)__double_as_longlong(v); }
     __device__ double value() const { return v; }
+    static __device__ double value_of(uint64_t b) { return __longlong_as_double((long long)b); }
 };
 
 template <>
@@ -694,6 +695,11 @@ struct Acc<1> {
     }
     __device__ uint64_t bits() const { return f2_pack(v); }
     __device__ double value() const { return (double)v.hi + (double)v.lo; }
+    static __device__ double value_of(uint64_t b)
+    {
+        const F2 x = f2_unpack(b);
+        return (double)x.hi + (double)x.lo;
+    }
 };
 
 /* Dependency of v: fold over DAG successors.  'ordered' sums them in
@@ -775,7 +781,10 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
     return acc;
 }
 
-/* Phase 3: dependencies, the levels in reverse, and the mode's output. */
+/* Phase 3: dependencies, the levels in reverse, then the mode's output in
+ * one pass in vertex order: the levels visit vertices scattered over the
+ * view, while the output pass reads the dependencies back with coalesced
+ * loads and writes (or atomically adds) whole lines. */
 template <int MODE, int PREC, bool ORDERED, class T>
 __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const View &V, const Slot S, const T &t,
                                    uint32_t task, uint32_t c, uint32_t src, uint32_t nlev)
@@ -789,20 +798,25 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
             if (v == src) continue;
             const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC>(V, S, v);
             S.sd[2 * (uint64_t)v + 1] = acc.bits();
-            const double dv = acc.value();
-            if (MODE == MODE_DEP) {
-                p.out_dep[(uint64_t)task * V.n + v] = dv;
-            } else if (MODE == MODE_EXACT) {
-                if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
-            } else if (MODE == MODE_PIVOT) {
-                const double x = dv * wgt;
-                if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
-            } else if (MODE == MODE_CLUSTER) {
-                p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
-            }
         }
         t.sync();
         if (t.aborted()) return;
+    }
+    /* the source and unreached vertices have no dependency (their delta
+     * slots hold older values) */
+    for (uint32_t v = r0; v < V.n; v += R) {
+        const bool dep = v != src && dist_final(S, v) != DINF;
+        const double dv = dep ? Acc<PREC>::value_of(S.sd[2 * (uint64_t)v + 1]) : 0.0;
+        if (MODE == MODE_DEP) {
+            p.out_dep[(uint64_t)task * V.n + v] = dv;
+        } else if (MODE == MODE_EXACT) {
+            if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
+        } else if (MODE == MODE_PIVOT) {
+            const double x = dv * wgt;
+            if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
+        } else if (MODE == MODE_CLUSTER) {
+            p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
+        }
     }
 }
 

@@ -8,6 +8,7 @@
 #include "fbc_gpu.h"
 #include "fbc_internal.h"
 #include "fbc_kmeans.h"
+#include "fbc_classes.h"
 
 #include <algorithm>
 #include <cmath>
@@ -28,6 +29,7 @@ struct fbc_ctx {
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
     uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
+    uint64_t class_key_mask = ~0ull;  /* class hashes kept (fewer bits: collisions, a test hook) */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
     uint64_t plain_fallbacks = 0;     /* plain-form runs finished on the batched kernel */
     int cu_count = 0;
@@ -72,6 +74,13 @@ struct fbc_clusters {
              *d_vcl = nullptr, *d_rp = nullptr, *d_rrp = nullptr;
     uint64_t *d_rpoff = nullptr, *d_voff = nullptr, *d_vioff = nullptr, *d_doff = nullptr;
     uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
+    /* VertexInfo rows left on the device by fbc_cluster_eval_resident */
+    uint32_t *d_vil = nullptr;
+    int32_t *d_vic = nullptr;
+    /* classes of the last fbc_cluster_classes (device, in class order) */
+    uint64_t ncls = 0;
+    std::vector<uint32_t> cls_ptr, cls_size;
+    uint32_t *d_cls_cluster = nullptr, *d_cls_grow = nullptr, *d_cls_ptr = nullptr, *d_rmin = nullptr;
 };
 
 namespace {
@@ -331,6 +340,8 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
     else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
     else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
+    else if (!std::strcmp(name, "class_key_bits"))
+        ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
     return 0;
 }
@@ -785,15 +796,25 @@ void fbc_clusters_free(fbc_clusters *cl)
     dfree(cl->d_voff);
     dfree(cl->d_vioff);
     dfree(cl->d_doff);
+    dfree(cl->d_vil);
+    dfree(cl->d_vic);
+    dfree(cl->d_cls_cluster);
+    dfree(cl->d_cls_grow);
+    dfree(cl->d_cls_ptr);
+    dfree(cl->d_rmin);
     delete cl;
 }
 
-int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uint32_t *vi_len,
-                     int32_t *vi_cnt)
+static int cluster_eval(fbc_ctx *ctx, fbc_clusters *cl, double *intra_bc, uint32_t *vi_len, int32_t *vi_cnt,
+                        bool resident)
 {
-    if (!ctx || !cl || !intra_bc || (cl->vi_total && (!vi_len || !vi_cnt)))
-        return fail(ctx, FBC_E_ARG, "null argument");
     const fbc_graph *g = cl->g;
+    /* a new evaluation replaces the rows and classes of an earlier one */
+    dfree(cl->d_vil);
+    dfree(cl->d_vic);
+    cl->ncls = 0;
+    cl->cls_ptr.clear();
+    cl->cls_size.clear();
     double *d_intra = nullptr, *d_dmat = nullptr;
     uint32_t *d_vil = nullptr;
     int32_t *d_vic = nullptr;
@@ -850,8 +871,7 @@ int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uin
             dfree(d_taux);
             if ((e = upload(&d_tsrc, tsrc, ctx->stream)) != hipSuccess ||
                 (e = upload(&d_taux, taux, ctx->stream)) != hipSuccess ||
-                (e = hipMemcpy(cl->d_doff, doff.data(), doff.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-                (e = hipMemsetAsync(d_dmat, 0, used * 8, ctx->stream)) != hipSuccess) {
+                (e = hipMemcpy(cl->d_doff, doff.data(), doff.size() * 8, hipMemcpyHostToDevice)) != hipSuccess) {
                 rc = hip_fail(ctx, e, "cluster chunk setup");
                 break;
             }
@@ -892,9 +912,17 @@ int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uin
     }
     if (!rc) {
         if ((e = hipMemcpy(intra_bc, d_intra, (size_t)g->n * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
-            (cl->vi_total && (e = hipMemcpy(vi_len, d_vil, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess) ||
-            (cl->vi_total && (e = hipMemcpy(vi_cnt, d_vic, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess))
+            (!resident && cl->vi_total &&
+             (e = hipMemcpy(vi_len, d_vil, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess) ||
+            (!resident && cl->vi_total &&
+             (e = hipMemcpy(vi_cnt, d_vic, cl->vi_total * 4, hipMemcpyDeviceToHost)) != hipSuccess))
             rc = hip_fail(ctx, e, "result download");
+    }
+    if (!rc && resident) {
+        cl->d_vil = d_vil;
+        cl->d_vic = d_vic;
+        d_vil = nullptr;
+        d_vic = nullptr;
     }
     dfree(d_intra);
     dfree(d_dmat);
@@ -904,6 +932,247 @@ int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uin
     dfree(d_taux);
     ctx->last_ms = ms;
     ctx->last_te = te;
+    return rc;
+}
+
+int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc, uint32_t *vi_len,
+                     int32_t *vi_cnt)
+{
+    if (!ctx || !cl || !intra_bc || (cl->vi_total && (!vi_len || !vi_cnt)))
+        return fail(ctx, FBC_E_ARG, "null argument");
+    return cluster_eval(ctx, const_cast<fbc_clusters *>(cl), intra_bc, vi_len, vi_cnt, false);
+}
+
+int fbc_cluster_eval_resident(fbc_ctx *ctx, fbc_clusters *cl, double *intra_bc)
+{
+    if (!ctx || !cl || !intra_bc) return fail(ctx, FBC_E_ARG, "null argument");
+    return cluster_eval(ctx, cl, intra_bc, nullptr, nullptr, true);
+}
+
+int fbc_cluster_rows(fbc_ctx *ctx, const fbc_clusters *cl, uint32_t *vi_len, int32_t *vi_cnt)
+{
+    if (!ctx || !cl || (cl->vi_total && (!vi_len || !vi_cnt))) return fail(ctx, FBC_E_ARG, "null argument");
+    if (cl->vi_total && !cl->d_vil) return fail(ctx, FBC_E_ARG, "no resident VertexInfo rows");
+    if (!cl->vi_total) return 0;
+    hipError_t e = hipMemcpy(vi_len, cl->d_vil, cl->vi_total * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(vi_cnt, cl->d_vic, cl->vi_total * 4, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "row download");
+}
+
+namespace {
+
+fbcdev::RowSet row_set(const fbc_clusters *cl)
+{
+    fbcdev::RowSet rs;
+    rs.nrows = cl->voff.empty() ? 0 : cl->voff.back() + cl->n.back();
+    rs.ncl = cl->ncl;
+    rs.vcl = cl->d_vcl;
+    rs.voff = cl->d_voff;
+    rs.bptr = cl->d_bptr;
+    rs.blocal = cl->d_blocal;
+    rs.vioff = cl->d_vioff;
+    rs.vi_len = cl->d_vil;
+    rs.vi_cnt = cl->d_vic;
+    rs.gid = cl->d_gid;
+    return rs;
+}
+
+} // namespace
+
+int fbc_cluster_classes(fbc_ctx *ctx, fbc_clusters *cl, const double *intra_bc, uint32_t *cls_ptr,
+                        uint32_t *piv_row, uint32_t *weight)
+{
+    if (!ctx || !cl || !intra_bc || !cls_ptr || !piv_row || !weight) return fail(ctx, FBC_E_ARG, "null argument");
+    if (cl->vi_total && !cl->d_vil) return fail(ctx, FBC_E_ARG, "no resident VertexInfo rows");
+    const fbcdev::RowSet rs = row_set(cl);
+    const uint64_t n = rs.nrows;
+    if (n >= 0x7FFFFFFFull) return fail(ctx, FBC_E_UNSUPPORTED, "too many cluster vertices for the class sort");
+    dfree(cl->d_cls_cluster);
+    dfree(cl->d_cls_grow);
+    dfree(cl->d_cls_ptr);
+    dfree(cl->d_rmin);
+    cl->ncls = 0;
+    cl->cls_ptr.clear();
+    cl->cls_size.clear();
+    uint64_t *d_key = nullptr, *d_skey = nullptr;
+    uint32_t *d_val = nullptr, *d_sval = nullptr, *d_head = nullptr, *d_size = nullptr, *d_piv = nullptr,
+             *d_pos = nullptr, *d_crow = nullptr, *d_csize = nullptr, *d_flag = nullptr;
+    uint8_t *d_isb = nullptr;
+    double *d_intra = nullptr;
+    void *d_tmp = nullptr;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    hipError_t e;
+    int rc = 0;
+    if ((e = fbcdev::classes_sort_bytes(n, &sort_bytes)) != hipSuccess ||
+        (e = fbcdev::classes_scan_bytes(n, &scan_bytes)) != hipSuccess ||
+        (e = dalloc(&d_key, n)) != hipSuccess || (e = dalloc(&d_skey, n)) != hipSuccess ||
+        (e = dalloc(&d_val, n)) != hipSuccess || (e = dalloc(&d_sval, n)) != hipSuccess ||
+        (e = dalloc(&cl->d_rmin, n)) != hipSuccess || (e = dalloc(&d_head, n)) != hipSuccess ||
+        (e = dalloc(&d_size, n)) != hipSuccess || (e = dalloc(&d_piv, n)) != hipSuccess ||
+        (e = dalloc(&d_pos, n)) != hipSuccess || (e = dalloc(&d_isb, n)) != hipSuccess ||
+        (e = dalloc(&d_flag, 1)) != hipSuccess ||
+        (e = dalloc((uint8_t **)&d_tmp, std::max(sort_bytes, scan_bytes))) != hipSuccess ||
+        (e = upload_n(&d_intra, intra_bc, cl->g->n, ctx->stream)) != hipSuccess)
+        rc = hip_fail(ctx, e, "class setup");
+    uint32_t collision = 0;
+    if (!rc) {
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        if ((e = fbcdev::classes_launch_hash(rs, ctx->class_key_mask, d_key, d_val, cl->d_rmin, ctx->stream)) !=
+                hipSuccess ||
+            (e = fbcdev::classes_launch_borders(rs, d_isb, ctx->stream)) != hipSuccess ||
+            (e = fbcdev::classes_sort(d_tmp, sort_bytes, d_key, d_skey, d_val, d_sval, n, ctx->stream)) !=
+                hipSuccess ||
+            (e = fbcdev::classes_launch_runs(rs, d_skey, d_sval, cl->d_rmin, d_isb, d_intra, d_head, d_size, d_piv,
+                                             d_flag, ctx->stream)) != hipSuccess ||
+            (e = fbcdev::classes_scan(d_tmp, scan_bytes, d_head, d_pos, n, ctx->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(&collision, d_flag, 4, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+            rc = hip_fail(ctx, e, "class grouping");
+    }
+    if (!rc && collision) rc = fail(ctx, FBC_E_UNSUPPORTED, "hash collision in the class grouping (use the host selector)");
+    uint32_t total = 0;
+    if (!rc) {
+        uint32_t last[2] = {0, 0};
+        if (n && ((e = hipMemcpy(&last[0], d_pos + n - 1, 4, hipMemcpyDeviceToHost)) != hipSuccess ||
+                  (e = hipMemcpy(&last[1], d_head + n - 1, 4, hipMemcpyDeviceToHost)) != hipSuccess))
+            rc = hip_fail(ctx, e, "class count");
+        total = last[0] + last[1];
+    }
+    if (!rc) {
+        if ((e = dalloc(&d_crow, total)) != hipSuccess || (e = dalloc(&d_csize, total)) != hipSuccess ||
+            (e = dalloc(&cl->d_cls_cluster, total)) != hipSuccess || (e = dalloc(&cl->d_cls_grow, total)) != hipSuccess ||
+            (e = dalloc(&cl->d_cls_ptr, (size_t)cl->ncl + 1)) != hipSuccess ||
+            (e = fbcdev::classes_launch_compact(rs, d_head, d_pos, d_size, d_piv, d_crow, d_csize, cl->d_cls_cluster,
+                                                cl->d_cls_grow, cl->d_cls_ptr, ctx->stream)) != hipSuccess ||
+            (e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess ||
+            (e = hipMemcpy(cls_ptr, cl->d_cls_ptr, ((size_t)cl->ncl + 1) * 4, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (total && (e = hipMemcpy(piv_row, d_crow, (size_t)total * 4, hipMemcpyDeviceToHost)) != hipSuccess) ||
+            (total && (e = hipMemcpy(weight, d_csize, (size_t)total * 4, hipMemcpyDeviceToHost)) != hipSuccess))
+            rc = hip_fail(ctx, e, "class list");
+    }
+    if (!rc) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        ctx->last_ms = ms;
+        ctx->last_te = 0;
+        cl->ncls = total;
+        cl->cls_ptr.assign(cls_ptr, cls_ptr + cl->ncl + 1);
+        cl->cls_size.assign(weight, weight + total);
+    } else {
+        dfree(cl->d_cls_cluster);
+        dfree(cl->d_cls_grow);
+        dfree(cl->d_cls_ptr);
+        dfree(cl->d_rmin);
+    }
+    for (auto *q : {d_key, d_skey}) dfree(q);
+    for (auto *q : {d_val, d_sval, d_head, d_size, d_piv, d_pos, d_crow, d_csize, d_flag}) dfree(q);
+    dfree(d_isb);
+    dfree(d_intra);
+    if (d_tmp) (void)hipFree(d_tmp);
+    return rc ? rc : (int)0;
+}
+
+int fbc_cluster_class_features(fbc_ctx *ctx, const fbc_clusters *cl, uint32_t c, int32_t *features)
+{
+    if (!ctx || !cl || !features) return fail(ctx, FBC_E_ARG, "null argument");
+    if (c >= cl->ncl || cl->cls_ptr.empty()) return fail(ctx, FBC_E_ARG, "no classes for this cluster");
+    const uint64_t P = cl->cls_ptr[c + 1] - cl->cls_ptr[c], D = 2ull * (cl->bptr[c + 1] - cl->bptr[c]);
+    if (!P || !D) return 0;
+    std::vector<uint64_t> xoff(cl->ncl, ~uint64_t(0));
+    xoff[c] = 0;
+    uint64_t *d_xoff = nullptr;
+    int32_t *d_X = nullptr;
+    hipError_t e;
+    int rc = 0;
+    if ((e = upload(&d_xoff, xoff, ctx->stream)) != hipSuccess || (e = dalloc(&d_X, P * D)) != hipSuccess ||
+        (e = fbcdev::classes_launch_gather(row_set(cl), cl->ncls, cl->d_cls_cluster, cl->d_cls_grow, cl->d_rmin,
+                                           cl->d_cls_ptr, d_xoff, d_X, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess ||
+        (e = hipMemcpy(features, d_X, P * D * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(ctx, e, "class features");
+    dfree(d_xoff);
+    dfree(d_X);
+    return rc;
+}
+
+static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
+                         const uint32_t *k, const int32_t *dX, const std::vector<uint64_t> &xoff,
+                         const int32_t *dW, double min_variance, uint32_t max_iteration, uint32_t *centroids,
+                         int32_t *cweights);
+
+int fbc_cluster_kmeans(fbc_ctx *ctx, const fbc_clusters *cl, double kfrac, double min_variance,
+                       uint32_t max_iteration, uint32_t *kptr, uint32_t *centroids, int32_t *cweights)
+{
+    if (!ctx || !cl || !kptr || !centroids || !cweights) return fail(ctx, FBC_E_ARG, "null argument");
+    if (cl->cls_ptr.empty()) return fail(ctx, FBC_E_ARG, "no classes: call fbc_cluster_classes first");
+    const uint32_t ncl = cl->ncl;
+    /* the largest |feature| of every cluster decides where its k-means runs */
+    unsigned long long *d_fmax = nullptr;
+    std::vector<unsigned long long> fmax(ncl ? ncl : 1);
+    hipError_t e;
+    int rc = 0;
+    if ((e = dalloc(&d_fmax, ncl)) != hipSuccess ||
+        (e = fbcdev::classes_launch_feature_max(row_set(cl), cl->ncls, cl->d_cls_cluster, cl->d_cls_grow,
+                                                cl->d_rmin, d_fmax, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess ||
+        (ncl && (e = hipMemcpy(fmax.data(), d_fmax, ncl * sizeof(unsigned long long), hipMemcpyDeviceToHost)) !=
+                    hipSuccess))
+        rc = hip_fail(ctx, e, "k-means feature range");
+    dfree(d_fmax);
+    if (rc) return rc;
+    std::vector<uint32_t> sets, P, D, K;
+    std::vector<uint64_t> xoff, xoff_cl(ncl, ~uint64_t(0));
+    std::vector<uint32_t> setk(ncl, 0);
+    uint64_t xs = 0;
+    for (uint32_t c = 0; c < ncl; ++c) {
+        const uint32_t pc = cl->cls_ptr[c + 1] - cl->cls_ptr[c], nb = cl->bptr[c + 1] - cl->bptr[c];
+        if (!pc || !nb) continue;
+        const uint32_t kc = (uint32_t)std::max((int)(pc * kfrac), 1);
+        const double mx = (double)fmax[c];
+        if (2.0 * 2 * nb * mx * mx >= 9007199254740992.0 || kc > FBC_KMEANS_MAX_K || kc > pc) continue; /* host */
+        sets.push_back(c);
+        P.push_back(pc);
+        D.push_back(2 * nb);
+        K.push_back(kc);
+        xoff.push_back(xs);
+        xoff_cl[c] = xs;
+        setk[c] = kc;
+        xs += (uint64_t)pc * 2 * nb;
+    }
+    kptr[0] = 0;
+    for (uint32_t c = 0; c < ncl; ++c) kptr[c + 1] = kptr[c] + setk[c];
+    if (sets.empty()) return 0;
+    /* weights: class sizes, in set order */
+    std::vector<int32_t> W;
+    uint64_t *d_xoff = nullptr;
+    int32_t *d_X = nullptr, *d_W = nullptr;
+    std::vector<uint32_t> oc(kptr[ncl]);
+    std::vector<int32_t> ow(kptr[ncl]);
+    rc = 0;
+    if ((e = upload(&d_xoff, xoff_cl, ctx->stream)) != hipSuccess || (e = dalloc(&d_X, xs)) != hipSuccess ||
+        (e = fbcdev::classes_launch_gather(row_set(cl), cl->ncls, cl->d_cls_cluster, cl->d_cls_grow, cl->d_rmin,
+                                           cl->d_cls_ptr, d_xoff, d_X, ctx->stream)) != hipSuccess)
+        rc = hip_fail(ctx, e, "k-means points");
+    if (!rc) {
+        for (uint32_t c : sets)
+            W.insert(W.end(), cl->cls_size.begin() + cl->cls_ptr[c], cl->cls_size.begin() + cl->cls_ptr[c + 1]);
+        if ((e = upload(&d_W, W, ctx->stream)) != hipSuccess) rc = hip_fail(ctx, e, "k-means weights");
+    }
+    if (!rc)
+        rc = kmeans_device(ctx, (uint32_t)sets.size(), P.data(), D.data(), K.data(), d_X, xoff, d_W, min_variance,
+                           max_iteration, oc.data(), ow.data());
+    if (!rc) {
+        size_t o = 0;
+        for (size_t i = 0; i < sets.size(); ++i) {
+            std::copy(oc.begin() + o, oc.begin() + o + K[i], centroids + kptr[sets[i]]);
+            std::copy(ow.begin() + o, ow.begin() + o + K[i], cweights + kptr[sets[i]]);
+            o += K[i];
+        }
+    }
+    dfree(d_xoff);
+    dfree(d_X);
+    dfree(d_W);
     return rc;
 }
 
@@ -990,37 +1259,24 @@ int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *
     return pivot_phase(ctx, cl, pivot, weight, pivot_cluster, npiv, intra_bc, FBC_PREC_F64, bc, true);
 }
 
-int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
-                     const uint32_t *k, const int32_t *features, const int32_t *weights, double min_variance,
-                     uint32_t max_iteration, uint32_t *centroids, int32_t *cweights)
+/* k-means of sets whose features (dX, set s at xoff[s]) and weights (dW,
+ * concatenated in set order) are already in device memory; sizes checked by
+ * the caller. */
+static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
+                         const uint32_t *k, const int32_t *dX, const std::vector<uint64_t> &xoff,
+                         const int32_t *dW, double min_variance, uint32_t max_iteration, uint32_t *centroids,
+                         int32_t *cweights)
 {
-    if (!ctx || (nsets && (!npoints || !nfeat || !k || !features || !weights || !centroids || !cweights)))
-        return fail(ctx, FBC_E_ARG, "null argument");
-    if (nsets == 0) return 0;
-    std::vector<uint64_t> xoff(nsets), poff(nsets), koff(nsets), moff(nsets), goff(nsets, 0);
-    uint64_t xs = 0, ps = 0, ks = 0, ms = 0;
+    std::vector<uint64_t> poff(nsets), koff(nsets), moff(nsets), goff(nsets, 0);
+    uint64_t ps = 0, ks = 0, ms = 0;
     for (uint32_t s = 0; s < nsets; ++s) {
-        if (npoints[s] == 0 || k[s] == 0 || k[s] > npoints[s]) return fail(ctx, FBC_E_ARG, "bad k-means set sizes");
-        if (k[s] > FBC_KMEANS_MAX_K) return fail(ctx, FBC_E_UNSUPPORTED, "k-means set with too many centroids");
-        xoff[s] = xs;
         poff[s] = ps;
         koff[s] = ks;
         moff[s] = ms;
-        xs += (uint64_t)npoints[s] * nfeat[s];
         ps += npoints[s];
         ks += k[s];
         ms += (uint64_t)k[s] * nfeat[s];
     }
-    /* exactness precondition (the squared norms come from the Gram diagonal) */
-    int bad = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
-    for (uint32_t s = 0; s < nsets; ++s) {
-        double mx = 0;
-        const int32_t *X = features + xoff[s];
-        for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) mx = std::max(mx, std::fabs((double)X[i]));
-        if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0) bad |= 2;
-    }
-    if (bad & 2) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
     /* groups of sets whose Gram matrices fit the budget together */
     const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30), 1);
     std::vector<std::vector<uint32_t>> groups;
@@ -1046,10 +1302,9 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     p.min_variance = min_variance;
     uint32_t *dP = nullptr, *dD = nullptr, *dK = nullptr, *dtiles = nullptr;
     uint64_t *dxo = nullptr, *dpo = nullptr, *dko = nullptr, *dmo = nullptr, *dgo = nullptr;
-    int32_t *dX = nullptr;
     double *dxn = nullptr, *dcd = nullptr, *dmd = nullptr, *dmsum = nullptr, *dG = nullptr,
            *dkv = nullptr;
-    int32_t *dW = nullptr, *dcw = nullptr;
+    int32_t *dcw = nullptr;
     uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
              *dmem = nullptr, *dact = nullptr;
     hipError_t e;
@@ -1059,8 +1314,7 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
         (e = upload(&dK, K, ctx->stream)) != hipSuccess || (e = upload(&dxo, xoff, ctx->stream)) != hipSuccess ||
         (e = upload(&dpo, poff, ctx->stream)) != hipSuccess || (e = upload(&dko, koff, ctx->stream)) != hipSuccess ||
         (e = upload(&dmo, moff, ctx->stream)) != hipSuccess || (e = upload(&dgo, goff, ctx->stream)) != hipSuccess ||
-        (e = upload_n(&dX, features, xs, ctx->stream)) != hipSuccess || (e = dalloc(&dxn, ps)) != hipSuccess ||
-        (e = upload_n(&dW, weights, ps, ctx->stream)) != hipSuccess || (e = upload(&dact, active, ctx->stream)) != hipSuccess ||
+        (e = dalloc(&dxn, ps)) != hipSuccess || (e = upload(&dact, active, ctx->stream)) != hipSuccess ||
         (e = dalloc(&dcd, ps)) != hipSuccess || (e = dalloc(&dmd, ps)) != hipSuccess ||
         (e = dalloc(&dkv, ks)) != hipSuccess ||
         (e = dalloc(&dmsum, std::max<uint64_t>(ms, 1))) != hipSuccess || (e = dalloc(&dcw, ks)) != hipSuccess ||
@@ -1174,12 +1428,50 @@ int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, cons
     }
     for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
     for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
-    dfree(dX);
     for (auto *q : {dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
-    dfree(dW);
     dfree(dcw);
     ctx->last_ms = ms_total;
     ctx->last_te = 0;
+    return rc;
+}
+
+int fbc_kmeans_batch(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, const uint32_t *nfeat,
+                     const uint32_t *k, const int32_t *features, const int32_t *weights, double min_variance,
+                     uint32_t max_iteration, uint32_t *centroids, int32_t *cweights)
+{
+    if (!ctx || (nsets && (!npoints || !nfeat || !k || !features || !weights || !centroids || !cweights)))
+        return fail(ctx, FBC_E_ARG, "null argument");
+    if (nsets == 0) return 0;
+    std::vector<uint64_t> xoff(nsets);
+    uint64_t xs = 0, ps = 0;
+    for (uint32_t s = 0; s < nsets; ++s) {
+        if (npoints[s] == 0 || k[s] == 0 || k[s] > npoints[s]) return fail(ctx, FBC_E_ARG, "bad k-means set sizes");
+        if (k[s] > FBC_KMEANS_MAX_K) return fail(ctx, FBC_E_UNSUPPORTED, "k-means set with too many centroids");
+        xoff[s] = xs;
+        xs += (uint64_t)npoints[s] * nfeat[s];
+        ps += npoints[s];
+    }
+    /* exactness precondition (the squared norms come from the Gram diagonal) */
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    for (uint32_t s = 0; s < nsets; ++s) {
+        double mx = 0;
+        const int32_t *X = features + xoff[s];
+        for (uint64_t i = 0; i < (uint64_t)npoints[s] * nfeat[s]; ++i) mx = std::max(mx, std::fabs((double)X[i]));
+        if (2.0 * nfeat[s] * mx * mx >= 9007199254740992.0) bad |= 2;
+    }
+    if (bad & 2) return fail(ctx, FBC_E_UNSUPPORTED, "k-means features too large for exact FP64 distances");
+    int32_t *dX = nullptr, *dW = nullptr;
+    hipError_t e;
+    int rc = 0;
+    if ((e = upload_n(&dX, features, xs, ctx->stream)) != hipSuccess ||
+        (e = upload_n(&dW, weights, ps, ctx->stream)) != hipSuccess)
+        rc = hip_fail(ctx, e, "k-means upload");
+    if (!rc)
+        rc = kmeans_device(ctx, nsets, npoints, nfeat, k, dX, xoff, dW, min_variance, max_iteration, centroids,
+                           cweights);
+    dfree(dX);
+    dfree(dW);
     return rc;
 }
 

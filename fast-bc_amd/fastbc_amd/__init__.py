@@ -25,6 +25,8 @@ EXPORTS = (
     "fbc_exact_bc", "fbc_clusters_create", "fbc_clusters_borders", "fbc_clusters_edges",
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
     "fbc_last_counters", "fbc_kmeans_batch", "fbc_pivot_phase_plain", "fbc_set_option",
+    "fbc_cluster_eval_resident", "fbc_cluster_rows", "fbc_cluster_classes", "fbc_cluster_kmeans",
+    "fbc_cluster_class_features",
 )
 
 # Every symbol include/fbc_host.h declares.
@@ -81,6 +83,12 @@ def lib():
         L.fbc_clusters_edges.restype = ctypes.c_int64
         L.fbc_clusters_free.argtypes = [_vp]
         L.fbc_cluster_eval.argtypes = [_vp, _vp, _f64p, _u32p, _i32p]
+        L.fbc_cluster_eval_resident.argtypes = [_vp, _vp, _f64p]
+        L.fbc_cluster_rows.argtypes = [_vp, _vp, _u32p, _i32p]
+        L.fbc_cluster_classes.argtypes = [_vp, _vp, _f64p, _u32p, _u32p, _u32p]
+        L.fbc_cluster_kmeans.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _u32p,
+                                         _u32p, _i32p]
+        L.fbc_cluster_class_features.argtypes = [_vp, _vp, ctypes.c_uint32, _i32p]
         L.fbc_pivot_phase.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p,
                                       ctypes.c_int, _f64p]
         L.fbc_pivot_phase_plain.argtypes = [_vp, _vp, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p]
@@ -303,6 +311,50 @@ class Clusters:
         C = np.zeros(tot, np.int32)
         self.graph.ctx.check(lib().fbc_cluster_eval(self.graph.ctx.h, self.h, bc, L, C))
         return bc, L, C
+
+    def evaluate_resident(self, intra_bc=None):
+        """fbc_cluster_eval_resident: intra-cluster BC; the VertexInfo rows stay on the device."""
+        n = self.graph.n
+        bc = np.zeros(n) if intra_bc is None else np.ascontiguousarray(intra_bc, np.float64).copy()
+        self.graph.ctx.check(lib().fbc_cluster_eval_resident(self.graph.ctx.h, self.h, bc))
+        return bc
+
+    def rows(self):
+        """The resident VertexInfo rows (fbc_cluster_rows), flattened like evaluate()'s."""
+        tot = max(int(self.vi_off[-1]), 1)
+        L = np.zeros(tot, np.uint32)
+        C = np.zeros(tot, np.int32)
+        self.graph.ctx.check(lib().fbc_cluster_rows(self.graph.ctx.h, self.h, L, C))
+        return L, C
+
+    def classes(self, intra_bc):
+        """fbc_cluster_classes: (cls_ptr, pivot row within its cluster, class size)."""
+        cap = max(int(self.ptr[-1]), 1)
+        ptr = np.zeros(self.ncl + 1, np.uint32)
+        row = np.zeros(cap, np.uint32)
+        w = np.zeros(cap, np.uint32)
+        self.graph.ctx.check(lib().fbc_cluster_classes(self.graph.ctx.h, self.h,
+                                                         np.ascontiguousarray(intra_bc, np.float64), ptr, row, w))
+        t = int(ptr[-1])
+        return ptr, row[:t].copy(), w[:t].copy()
+
+    def kmeans(self, kfrac, min_variance=0.0, max_iteration=100):
+        """fbc_cluster_kmeans over the last classes: (kptr, centroids, weights)."""
+        cap = max(int(self.ptr[-1]), 1)
+        kptr = np.zeros(self.ncl + 1, np.uint32)
+        cent = np.zeros(cap, np.uint32)
+        cw = np.zeros(cap, np.int32)
+        self.graph.ctx.check(lib().fbc_cluster_kmeans(self.graph.ctx.h, self.h, kfrac, min_variance, max_iteration,
+                                                        kptr, cent, cw))
+        t = int(kptr[-1])
+        return kptr, cent[:t].copy(), cw[:t].copy()
+
+    def class_features(self, c, nclasses):
+        """fbc_cluster_class_features: [nclasses x 2 * borders] int32 k-means points of cluster c."""
+        nb = int(self.border_ptr[c + 1] - self.border_ptr[c])
+        f = np.zeros(max(nclasses * 2 * nb, 1), np.int32)
+        self.graph.ctx.check(lib().fbc_cluster_class_features(self.graph.ctx.h, self.h, c, f))
+        return f[:nclasses * 2 * nb].reshape(nclasses, 2 * nb)
 
     def vi_rows(self, flat, c):
         nb = int(self.border_ptr[c + 1] - self.border_ptr[c])

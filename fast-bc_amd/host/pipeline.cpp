@@ -4,9 +4,12 @@
  *
  *   1. Louvain on the host (bit-identical partition);
  *   2. cluster views + every (cluster, source) Brandes in one persistent
- *      launch (fbc_cluster_eval): intra-cluster BC in the reference's FP64
- *      order, VertexInfo rows;
- *   3. pivot selection per cluster on the host (OpenMP over clusters);
+ *      launch (fbc_cluster_eval_resident): intra-cluster BC in the
+ *      reference's FP64 order, VertexInfo rows left in device memory;
+ *   3. pivot selection: exact classes of every cluster on the device from
+ *      the resident VertexInfo rows (fbc_cluster_classes), k-means over
+ *      them on the device (fbc_cluster_kmeans); the host selector takes over
+ *      on a hash collision, the host k-means for sets the device declines;
  *   4. every pivot's full-graph Brandes in one persistent launch
  *      (fbc_pivot_phase), the weighted accumulation and the intra-cluster
  *      correction in FP64.
@@ -21,6 +24,7 @@
 #include <cstdio>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <set>
 #include <stdexcept>
@@ -32,6 +36,28 @@ namespace {
 double now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+/* Repeated centroids are dropped keeping the first occurrence and its
+ * weight; prow / pw are replaced by the k-means pivots
+ * (KMeansPivotSelector.h:103-129). */
+void apply_centroids(uint32_t ncl, const std::vector<std::vector<int>> &cent, const std::vector<std::vector<int>> &cw,
+                     std::vector<std::vector<int>> &prow, std::vector<std::vector<int>> &pw)
+{
+    for (uint32_t c = 0; c < ncl; ++c) {
+        if (prow[c].empty()) continue;
+        std::vector<int> rows, w;
+        std::set<int> seen;
+        for (size_t i = 0; i < cent[c].size(); ++i) {
+            const int r = prow[c][cent[c][i]];
+            if (seen.insert(r).second) {
+                rows.push_back(r);
+                w.push_back(cw[c][i]);
+            }
+        }
+        prow[c].swap(rows);
+        pw[c].swap(w);
+    }
 }
 
 /* KMeansPivotSelector (KMeansPivotSelector.h:79-132) for every cluster:
@@ -112,23 +138,113 @@ void kmeans_stage(Device &dev, uint32_t ncl, const std::vector<uint32_t> &bptr, 
         kmeans(std::max((int)(P * kfrac), 1), P, (int)nb, plen[c].data(), pcnt[c].data(), pw[c].data(), 0.0, 100,
                cent[c], cw[c]);
     }
-    for (uint32_t c = 0; c < ncl; ++c) {
-        if (prow[c].empty()) continue;
-        std::vector<int> rows, w;
-        std::set<int> seen;
-        for (size_t i = 0; i < cent[c].size(); ++i) {
-            const int r = prow[c][cent[c][i]];
-            if (seen.insert(r).second) {
-                rows.push_back(r);
-                w.push_back(cw[c][i]);
-            }
-        }
-        prow[c].swap(rows);
-        pw[c].swap(w);
-    }
+    apply_centroids(ncl, cent, cw, prow, pw);
     t.kmeans = now() - t0;
     t.kmeans_gpu_sets = gsets.size();
     t.kmeans_host_sets = hsets.size();
+}
+
+/* KMeansPivotSelector on the classes the device grouped
+ * (fbc_cluster_kmeans); clusters it declines (features too large for exact
+ * FP64 distances, too many centroids) run the host k-means on their class
+ * rows. */
+void kmeans_device_stage(Device &dev, fbc_clusters *cl, uint32_t ncl, const std::vector<uint32_t> &bptr,
+                         double kfrac, std::vector<std::vector<int>> &prow, std::vector<std::vector<int>> &pw,
+                         PhaseTimes &t)
+{
+    const double t0 = now();
+    size_t total = 0;
+    for (uint32_t c = 0; c < ncl; ++c) total += prow[c].size();
+    std::vector<uint32_t> kptr((size_t)ncl + 1, 0), oc(std::max<size_t>(total, 1));
+    std::vector<int32_t> ow(std::max<size_t>(total, 1));
+    dev.check(fbc_cluster_kmeans(dev.ctx(), cl, kfrac, 0.0, 100, kptr.data(), oc.data(), ow.data()));
+    std::vector<std::vector<int>> cent(ncl), cw(ncl);
+    std::vector<uint32_t> hsets;
+    for (uint32_t c = 0; c < ncl; ++c) {
+        if (prow[c].empty()) continue;
+        if (kptr[c + 1] > kptr[c]) {
+            cent[c].assign(oc.begin() + kptr[c], oc.begin() + kptr[c + 1]);
+            cw[c].assign(ow.begin() + kptr[c], ow.begin() + kptr[c + 1]);
+        } else {
+            hsets.push_back(c);
+        }
+    }
+    if (kptr[ncl] > 0) fbc_last_stats(dev.ctx(), &t.kmeans_gpu_ms, nullptr);
+    /* the declined clusters' points, then their k-means in parallel */
+    std::vector<std::vector<double>> L(hsets.size());
+    std::vector<std::vector<int>> C(hsets.size());
+    for (size_t i = 0; i < hsets.size(); ++i) {
+        const uint32_t c = hsets[i], nb = bptr[c + 1] - bptr[c];
+        const size_t P = prow[c].size();
+        std::vector<int32_t> f(P * 2 * nb);
+        dev.check(fbc_cluster_class_features(dev.ctx(), cl, c, f.data()));
+        L[i].resize(P * nb);
+        C[i].resize(P * nb);
+        for (size_t e = 0; e < P * nb; ++e) {
+            L[i][e] = (double)f[2 * e];
+            C[i][e] = f[2 * e + 1];
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < hsets.size(); ++i) {
+        const uint32_t c = hsets[i], nb = bptr[c + 1] - bptr[c];
+        const int P = (int)prow[c].size();
+        kmeans(std::max((int)(P * kfrac), 1), P, (int)nb, L[i].data(), C[i].data(), pw[c].data(), 0.0, 100, cent[c],
+               cw[c]);
+    }
+    apply_centroids(ncl, cent, cw, prow, pw);
+    t.kmeans = now() - t0;
+    t.kmeans_gpu_sets = 0;
+    for (uint32_t c = 0; c < ncl; ++c) t.kmeans_gpu_sets += kptr[c + 1] > kptr[c];
+    t.kmeans_host_sets = hsets.size();
+}
+
+/* The host selector on rows copied out of the device: the fallback when the
+ * device's class grouping meets a hash collision. */
+void host_select(Device &dev, fbc_clusters *cl, uint32_t ncl, const std::vector<uint32_t> &cptr,
+                 const std::vector<uint32_t> &verts, const std::vector<uint32_t> &bptr,
+                 const std::vector<uint32_t> &borders, const std::vector<uint64_t> &vioff,
+                 const std::vector<double> &intra, double kfrac, std::vector<std::vector<int>> &prow,
+                 std::vector<std::vector<int>> &pw, PhaseTimes &t)
+{
+    /* uninitialised: the download writes every entry */
+    std::unique_ptr<uint32_t[]> vl(new uint32_t[std::max<uint64_t>(vioff[ncl], 1)]);
+    std::unique_ptr<int32_t[]> vc(new int32_t[std::max<uint64_t>(vioff[ncl], 1)]);
+    dev.check(fbc_cluster_rows(dev.ctx(), cl, vl.get(), vc.get()));
+    std::vector<std::vector<double>> plen(ncl);
+    std::vector<std::vector<int>> pcnt(ncl);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t c = 0; c < ncl; ++c) {
+        const int nv = (int)(cptr[c + 1] - cptr[c]);
+        const int nb = (int)(bptr[c + 1] - bptr[c]);
+        if (nb == 0) continue;
+        std::vector<double> len((size_t)nv * nb);
+        std::vector<int> cnt((size_t)nv * nb);
+        for (size_t i = 0; i < (size_t)nv * nb; ++i) {
+            len[i] = (double)vl[vioff[c] + i];
+            cnt[i] = vc[vioff[c] + i];
+        }
+        std::vector<char> isb((size_t)nv, 0);
+        for (int j = 0; j < nb; ++j) {
+            const uint32_t b = borders[bptr[c] + j];
+            /* borders are ascending, like the cluster's vertex list */
+            const uint32_t *first = &verts[cptr[c]], *last = &verts[cptr[c + 1]];
+            isb[std::lower_bound(first, last, b) - first] = 1;
+        }
+        std::vector<double> bcr((size_t)nv);
+        for (int r = 0; r < nv; ++r) bcr[r] = intra[verts[cptr[c] + r]];
+        select_pivots(nv, nb, len.data(), cnt.data(), bcr.data(), isb.data(), prow[c], pw[c]);
+        if (kfrac > 0) {
+            plen[c].resize(prow[c].size() * nb);
+            pcnt[c].resize(prow[c].size() * nb);
+            for (size_t p = 0; p < prow[c].size(); ++p)
+                for (int j = 0; j < nb; ++j) {
+                    plen[c][p * nb + j] = len[(size_t)prow[c][p] * nb + j];
+                    pcnt[c][p * nb + j] = cnt[(size_t)prow[c][p] * nb + j];
+                }
+        }
+    }
+    if (kfrac > 0) kmeans_stage(dev, ncl, bptr, kfrac, prow, pw, plen, pcnt, t);
 }
 
 } // namespace
@@ -138,6 +254,31 @@ Device::Device(int device)
     const int rc = fbc_create(device, &ctx_);
     if (rc) throw std::runtime_error("fbc_create failed (status " + std::to_string(rc) +
                                      "): a gfx950 (MI355X) device is required");
+    /* FBC_OPTIONS="name=value,..." sets fbc_set_option knobs on the context */
+    if (const char *env = std::getenv("FBC_OPTIONS")) {
+        std::string all(env);
+        size_t at = 0;
+        while (at < all.size()) {
+            size_t end = all.find(',', at);
+            if (end == std::string::npos) end = all.size();
+            const std::string item = all.substr(at, end - at);
+            at = end + 1;
+            if (item.empty()) continue;
+            const size_t eq = item.find('=');
+            if (eq == std::string::npos) throw std::invalid_argument("FBC_OPTIONS entry without '=': " + item);
+            char *stop = nullptr;
+            const std::string val = item.substr(eq + 1);
+            const long long v = std::strtoll(val.c_str(), &stop, 10);
+            if (val.empty() || *stop) throw std::invalid_argument("FBC_OPTIONS value is not an integer: " + item);
+            const int orc = fbc_set_option(ctx_, item.substr(0, eq).c_str(), v);
+            if (orc) {
+                const std::string msg = fbc_last_error(ctx_);
+                fbc_destroy(ctx_);
+                ctx_ = nullptr;
+                throw std::invalid_argument("FBC_OPTIONS: " + msg);
+            }
+        }
+    }
 }
 
 Device::~Device()
@@ -242,12 +383,10 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     for (uint32_t c = 0; c < ncl; ++c)
         vioff[c + 1] = vioff[c] + (uint64_t)(cptr[c + 1] - cptr[c]) * (bptr[c + 1] - bptr[c]);
     const double t_views = now() - t0;
-    /* uninitialised: the evaluation writes every entry */
-    std::unique_ptr<uint32_t[]> vl(new uint32_t[std::max<uint64_t>(vioff[ncl], 1)]);
-    std::unique_ptr<int32_t[]> vc(new int32_t[std::max<uint64_t>(vioff[ncl], 1)]);
     run.intra.assign((size_t)g.n, 0.0);
     const double t_eval = now();
-    dev.check(fbc_cluster_eval(dev.ctx(), cl, run.intra.data(), vl.get(), vc.get()));
+    /* the VertexInfo rows stay on the device for the class selection */
+    dev.check(fbc_cluster_eval_resident(dev.ctx(), cl, run.intra.data()));
     fbc_last_stats(dev.ctx(), &run.t.cluster_kernel_ms, &run.t.cluster_te);
     run.t.cluster = now() - t0;
     {
@@ -258,58 +397,31 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
         report(1, buf);
     }
 
-    /* exact pivots per cluster (the selectors only read their own cluster);
-     * the normalized rows of the exact pivots are kept for k-means */
+    /* exact pivots per cluster (the selectors only read their own cluster),
+     * on the device; a hash collision there sends the selection to the host */
     t0 = now();
     std::vector<std::vector<int>> piv(ncl), pw(ncl), prow(ncl);
-    std::vector<std::vector<double>> plen(ncl);
-    std::vector<std::vector<int>> pcnt(ncl);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (uint32_t c = 0; c < ncl; ++c) {
-        const int nv = (int)(cptr[c + 1] - cptr[c]);
-        const int nb = (int)(bptr[c + 1] - bptr[c]);
-        if (nb == 0) continue;
-        std::vector<double> len((size_t)nv * nb);
-        std::vector<int> cnt((size_t)nv * nb);
-        for (size_t i = 0; i < (size_t)nv * nb; ++i) {
-            len[i] = (double)vl[vioff[c] + i];
-            cnt[i] = vc[vioff[c] + i];
+    std::vector<uint32_t> cls_ptr((size_t)ncl + 1, 0), cls_row(std::max<size_t>(verts.size(), 1)),
+        cls_w(std::max<size_t>(verts.size(), 1));
+    const int crc = fbc_cluster_classes(dev.ctx(), cl, run.intra.data(), cls_ptr.data(), cls_row.data(), cls_w.data());
+    if (crc != 0 && crc != FBC_E_UNSUPPORTED) dev.check(crc);
+    if (crc == 0) {
+        for (uint32_t c = 0; c < ncl; ++c) {
+            prow[c].assign(cls_row.begin() + cls_ptr[c], cls_row.begin() + cls_ptr[c + 1]);
+            pw[c].assign(cls_w.begin() + cls_ptr[c], cls_w.begin() + cls_ptr[c + 1]);
         }
-        std::vector<char> isb((size_t)nv, 0);
-        for (int j = 0; j < nb; ++j) {
-            const uint32_t b = borders[bptr[c] + j];
-            /* borders are ascending, like the cluster's vertex list */
-            const uint32_t *first = &verts[cptr[c]], *last = &verts[cptr[c + 1]];
-            isb[std::lower_bound(first, last, b) - first] = 1;
-        }
-        std::vector<double> bcr((size_t)nv);
-        for (int r = 0; r < nv; ++r) bcr[r] = run.intra[verts[cptr[c] + r]];
-        select_pivots(nv, nb, len.data(), cnt.data(), bcr.data(), isb.data(), prow[c], pw[c]);
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "exact pivot classes: %u on the device (%.3f s)", cls_ptr[ncl], now() - t0);
+        report(1, buf);
         if (opt.kfrac > 0) {
-            plen[c].resize(prow[c].size() * nb);
-            pcnt[c].resize(prow[c].size() * nb);
-            for (size_t p = 0; p < prow[c].size(); ++p)
-                for (int j = 0; j < nb; ++j) {
-                    plen[c][p * nb + j] = len[(size_t)prow[c][p] * nb + j];
-                    pcnt[c][p * nb + j] = cnt[(size_t)prow[c][p] * nb + j];
-                }
+            kmeans_device_stage(dev, cl, ncl, bptr, opt.kfrac, prow, pw, run.t);
+            std::snprintf(buf, sizeof buf, "k-means: %zu sets on the GPU, %zu on the host (%.3f s, GPU kernels %.3f s)",
+                          run.t.kmeans_gpu_sets, run.t.kmeans_host_sets, run.t.kmeans, run.t.kmeans_gpu_ms * 1e-3);
+            report(1, buf);
         }
-    }
-    {
-        size_t exact = 0;
-        for (uint32_t c = 0; c < ncl; ++c) exact += prow[c].size();
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "exact pivot classes: %zu (%.3f s)", exact, now() - t0);
-        report(1, buf);
-    }
-    if (opt.kfrac > 0) {
-        kmeans_stage(dev, ncl, bptr, opt.kfrac, prow, pw, plen, pcnt, run.t);
-        char buf[160];
-        std::snprintf(buf, sizeof buf,
-                      "k-means: %zu sets on the GPU, %zu on the host (%.3f s; feature rows %.3f s, GPU kernels %.3f s)",
-                      run.t.kmeans_gpu_sets, run.t.kmeans_host_sets, run.t.kmeans, run.t.kmeans_prep,
-                      run.t.kmeans_gpu_ms * 1e-3);
-        report(1, buf);
+    } else {
+        report(1, std::string("class grouping on the host: ") + fbc_last_error(dev.ctx()));
+        host_select(dev, cl, ncl, cptr, verts, bptr, borders, vioff, run.intra, opt.kfrac, prow, pw, run.t);
     }
     for (uint32_t c = 0; c < ncl; ++c)
         for (size_t p = 0; p < prow[c].size(); ++p) piv[c].push_back((int)verts[cptr[c] + prow[c][p]]);

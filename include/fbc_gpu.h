@@ -76,7 +76,9 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
  * locality; 0: on the graph as given), "scratch_mb" (device memory for the
  * sources in flight, in MiB; automatic: a third of the device, at most
- * 24 GiB).  No counterpart in the reference. */
+ * 24 GiB), "class_key_bits" (bits of the row hashes fbc_cluster_classes
+ * keeps; 0: all 64; a few bits force collisions, a test hook).  No
+ * counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
@@ -138,6 +140,42 @@ void fbc_clusters_free(fbc_clusters *cl);
  * Borderless clusters are skipped (FASTBC_BRANDES_CLUSTERED_IGNORE_UNCONNECTED). */
 int fbc_cluster_eval(fbc_ctx *ctx, const fbc_clusters *cl, double *intra_bc,
                      uint32_t *vi_len, int32_t *vi_cnt);
+
+/* fbc_cluster_eval with the VertexInfo rows kept in device memory with the
+ * clusters (for fbc_cluster_classes) instead of copied out; fbc_cluster_rows
+ * copies them out later.  Same rows, same intra_bc. */
+int fbc_cluster_eval_resident(fbc_ctx *ctx, fbc_clusters *cl, double *intra_bc);
+int fbc_cluster_rows(fbc_ctx *ctx, const fbc_clusters *cl, uint32_t *vi_len, int32_t *vi_cnt);
+
+/* VertexInfoPivotSelector::selectPivots (VertexInfoPivotSelector.h:28-127)
+ * for every bordered cluster, on the resident rows: the exact classes of
+ * vertices with equal normalized rows (lengths minus the row's minimum,
+ * counts), in order of first appearance; each class's pivot is its first
+ * non-border member with the smallest intra_bc (its first member when all
+ * are borders, as FASTBC_BRANDES_ENABLE_PIVOT_BORDER, main.cpp:5, allows) and
+ * its weight the class size.  Outputs: cls_ptr[ncl + 1] (classes of cluster
+ * c are [cls_ptr[c], cls_ptr[c+1])), piv_row (row of the pivot within its
+ * cluster's vertex list) and weight, up to one entry per cluster vertex.
+ * Classes are found through a 64-bit hash of each row, verified entry by
+ * entry; a hash collision returns FBC_E_UNSUPPORTED, and the caller then
+ * selects on the host from fbc_cluster_rows.  The classes stay on the device
+ * for fbc_cluster_kmeans. */
+int fbc_cluster_classes(fbc_ctx *ctx, fbc_clusters *cl, const double *intra_bc, uint32_t *cls_ptr,
+                        uint32_t *piv_row, uint32_t *weight);
+
+/* KMeansPivotSelector's k-means (KMeansPivotSelector.h:79-132 with
+ * PlusPlusKMeans, as fbc_kmeans_batch) over each cluster's classes from the
+ * last fbc_cluster_classes: k_c = max((int)(P_c * kfrac), 1) for the P_c
+ * classes of cluster c, points = the classes' normalized rows, weights = the
+ * class sizes.  Outputs kptr[ncl + 1] and, in [kptr[c], kptr[c+1]), the
+ * centroids (class indices within the cluster) and their weights.  A cluster
+ * with classes but an empty range needs the host k-means (its features are
+ * too large for exact FP64 distances or k_c > FBC_KMEANS_MAX_K); its points
+ * come from fbc_cluster_class_features: P_c rows of 2 * B_c int32 features
+ * (length minus the row minimum, count, per border). */
+int fbc_cluster_kmeans(fbc_ctx *ctx, const fbc_clusters *cl, double kfrac, double min_variance,
+                       uint32_t max_iteration, uint32_t *kptr, uint32_t *centroids, int32_t *cweights);
+int fbc_cluster_class_features(fbc_ctx *ctx, const fbc_clusters *cl, uint32_t c, int32_t *features);
 
 /* The pivot loop of ClusteredBrandeBC::computeBC (ClusteredBrandesBC.h:141-166):
  *   bc[v] += w_p * dep_p(v)                for every v,
