@@ -33,6 +33,12 @@
 namespace fbcdev {
 
 constexpr int HEAVY_DEG = 96;   /* out-degree handled by the whole workgroup */
+/* Tight-arc masks: bit j of sd[2v+1] (between the in-degree sweep and v's
+ * dependency) says whether out-arc j of v lies on a shortest path, for
+ * out-degrees up to MASK_ARCS; MASK_NONE marks wider rows, whose arcs are
+ * tested against the distances. */
+constexpr uint32_t MASK_ARCS = 63;
+constexpr uint64_t MASK_NONE = ~uint64_t(0);
 constexpr int HEAVY_CAP = 64;
 /* Per team: RB arcs of one vertex have their loads and atomics issued
  * together; the ordered fold keeps up to SUCC successors in registers.  The
@@ -94,8 +100,10 @@ __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~u
 /* Per-source state.  key[v] packs the tentative distance (high 32 bits) with
  * a tag (low 32 bits): during the distance phase the round in which v was
  * last queued, afterwards its count of unprocessed DAG predecessors.  sd[2v]
- * is sigma(v), sd[2v+1] delta(v) (a double, or a float pair, bit-cast), so a
- * DAG successor's (sigma, delta) share one 16-byte line segment. */
+ * is sigma(v); sd[2v+1] holds v's tight-arc mask from the in-degree sweep
+ * until the dependency phase replaces it by delta(v) (a double, or a float
+ * pair, bit-cast), so a DAG successor's (sigma, delta) share one 16-byte line
+ * segment. */
 struct Slot {
     uint64_t *key;
     uint32_t *fstamp, *qa, *qb, *fa, *fb;
@@ -544,9 +552,46 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
     }
 }
 
-/* Push sigma(u) along the DAG arcs among [e0, e1), RB at a time with the
- * loads and atomics of a block in flight together; a successor whose last
- * predecessor this was joins the level list. */
+/* Push sigma(u) along u's tight arcs, given by its mask (bit j: arc e0 + j),
+ * RB at a time; a successor whose last predecessor this was joins the level
+ * list. */
+template <class T>
+__device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint64_t su, uint32_t e0,
+                                                  uint64_t mask, uint32_t *tail, uint32_t *err)
+{
+    constexpr int SC = T::SC, RB = T::RB;
+    uint64_t m = mask;
+    while (m) {
+        uint32_t j[RB];
+        bool ok[RB];
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            ok[k] = m != 0;
+            j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : 0u;
+            m &= m - 1;
+        }
+        uint32_t w[RB];
+#pragma unroll
+        for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
+        uint64_t old[RB], cnt[RB];
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            old[k] = ok[k] ? aadd64<SC>(&S.sd[2 * (uint64_t)w[k]], su) : 0;
+            cnt[k] = ok[k] ? asub64<SC>(&S.key[w[k]], 1) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            if (!ok[k]) continue;
+            if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
+            if ((uint32_t)cnt[k] == 1u) S.qa[aadd32<SC>(tail, 1u)] = w[k];
+        }
+    }
+}
+
+/* Push sigma(u) along the DAG arcs among [e0, e1) (every step-th), testing
+ * each against the distances, RB at a time with the loads and atomics of a
+ * block in flight together; a successor whose last predecessor this was
+ * joins the level list. */
 template <class T>
 __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t du, uint64_t su,
                                            uint32_t e0, uint32_t e1, uint32_t step, uint32_t *tail,
@@ -576,10 +621,13 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
     }
 }
 
-/* Phase 2: DAG in-degrees, then the Kahn sweep with exact sigma.  Returns the
- * number of levels; the level list is S.qa, the level starts S.fa.  The
- * in-degree goes into the low half of key[v] (the round tags are done with);
- * the high half keeps the distance. */
+/* Phase 2: DAG in-degrees and tight-arc masks, then the Kahn sweep with
+ * exact sigma.  Returns the number of levels; the level list is S.qa, the
+ * level starts S.fa.  The in-degree goes into the low half of key[v] (the
+ * round tags are done with); the high half keeps the distance.  The sweep
+ * walks the vertices in order, so the distance tests it makes for the masks
+ * read neighbouring lines; the Kahn sweep and the dependency phase, which
+ * visit vertices scattered, then touch only the successors. */
 template <class T>
 __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t sort_cap, uint32_t *err, Counters &ct)
@@ -592,7 +640,24 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
         const uint32_t dv = dist_of<SC>(S, v);
         uint32_t cin = 0;
         if (dv != DINF) {
-            te += V.rp[v + 1] - V.rp[v];
+            const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
+            te += e1 - e0;
+            uint64_t mask = MASK_NONE;
+            if (e1 - e0 <= MASK_ARCS) {
+                mask = 0;
+                for (uint32_t eb = e0; eb < e1; eb += RB) {
+                    uint2 a[RB];
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) a[j] = eb + j < e1 ? V.arcs[eb + j] : make_uint2(0u, 0u);
+                    uint32_t dw[RB];
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) dw[j] = eb + j < e1 ? dist_of<SC>(S, a[j].x) : DINF;
+#pragma unroll
+                    for (int j = 0; j < RB; ++j)
+                        if (dw[j] != DINF && (uint64_t)dv + a[j].y == dw[j]) mask |= 1ull << (eb + j - e0);
+                }
+            }
+            S.sd[2 * (uint64_t)v + 1] = mask;
             if (v != src) {
                 const uint32_t r1 = V.rrp[v + 1];
                 for (uint32_t rb = V.rrp[v]; rb < r1; rb += RB) {
@@ -636,7 +701,11 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
                     continue;
                 }
             }
-            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, &c.tail, err);
+            const uint64_t mask = ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
+            if (mask != MASK_NONE)
+                push_sigma_masked<T>(V, S, sigma_of<SC>(S, u), e0, mask, &c.tail, err);
+            else
+                push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, &c.tail, err);
         }
         t.lsync();
         const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
@@ -704,15 +773,67 @@ struct Acc<1> {
 
 /* Dependency of v: fold over DAG successors.  'ordered' sums them in
  * descending (d(w), w).  Distances, sigma and the successors' deltas are
- * final here, so plain loads see them after the level barrier. */
+ * final here, so plain loads see them after the level barrier.  With a
+ * tight-arc mask the successors are known: d(w) = d(v) + w(v, w) and only
+ * their (sigma, delta) pairs are read. */
 template <int PREC, bool ORDERED, int RB, int SUCC_REG>
 __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v)
 {
     Acc<PREC> acc;
     const uint32_t dv = dist_final(S, v);
-    const uint64_t sv_int = S.sd[2 * (uint64_t)v];
+    const ulonglong2 own = sd_pair(S, v);
+    const uint64_t sv_int = own.x, mask = own.y;
     const double sv = (double)sv_int;
     const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
+    if (!ORDERED && mask != MASK_NONE) {
+        /* ascending arc order, RB successors' loads in flight together */
+        uint64_t m = mask;
+        while (m) {
+            uint32_t j[RB];
+            bool ok[RB];
+#pragma unroll
+            for (int k = 0; k < RB; ++k) {
+                ok[k] = m != 0;
+                j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : 0u;
+                m &= m - 1;
+            }
+            uint32_t w[RB];
+#pragma unroll
+            for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
+            ulonglong2 sw[RB];
+#pragma unroll
+            for (int k = 0; k < RB; ++k) sw[k] = ok[k] ? sd_pair(S, w[k]) : make_ulonglong2(0, 0);
+#pragma unroll
+            for (int k = 0; k < RB; ++k)
+                if (ok[k]) acc.add(sv_int, sv, sw[k].x, sw[k].y);
+        }
+        return acc;
+    }
+    if (ORDERED && mask != MASK_NONE && e1 - e0 <= (uint32_t)SUCC_REG) {
+        uint2 a[SUCC_REG];
+#pragma unroll
+        for (int j = 0; j < SUCC_REG; ++j)
+            a[j] = (mask >> j) & 1 ? V.arcs[e0 + j] : make_uint2(0u, 0u);
+        uint64_t key[SUCC_REG];
+#pragma unroll
+        for (int j = 0; j < SUCC_REG; ++j)
+            key[j] = (mask >> j) & 1 ? ((uint64_t)(dv + a[j].y) << 32) | a[j].x : 0;
+#pragma unroll
+        for (int r = 0; r < SUCC_REG; ++r) {
+            uint64_t best = 0;
+            int bj = -1;
+#pragma unroll
+            for (int j = 0; j < SUCC_REG; ++j)
+                if (key[j] > best) { best = key[j]; bj = j; }
+            if (bj < 0) break;
+#pragma unroll
+            for (int j = 0; j < SUCC_REG; ++j)
+                if (j == bj) key[j] = 0;
+            const ulonglong2 b = sd_pair(S, (uint32_t)best);
+            acc.add(sv_int, sv, b.x, b.y);
+        }
+        return acc;
+    }
     if (!ORDERED) {
         /* ascending arc order, RB successors' loads in flight together */
         for (uint32_t eb = e0; eb < e1; eb += RB) {
