@@ -158,10 +158,11 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
-    /* whole-graph slots (tens of MB each) stop at ~2 per CU: beyond that the
-     * slots' working sets only contend for L2 and MALL */
+    /* whole-graph slots (tens of MB each) are bounded by memory: throughput
+     * levels off near 3 per CU (about 40 GiB of 1M-vertex slots), where the
+     * memory system, not occupancy, limits the launch */
     const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20
-                                          : std::min<size_t>(ctx->total_mem / 3, (size_t)24 << 30);
+                                          : std::min<size_t>(ctx->total_mem / 4, (size_t)40 << 30);
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     /* the wavefront kernel's workgroups address WAVES slots each: hold whole
      * workgroups' worth */
