@@ -441,49 +441,68 @@ static void blocked_order(const fbc_graph *g, std::vector<uint32_t> &perm)
     static const uint32_t SIZES[] = {4096, 256, 16};
     const uint32_t n = g->n;
     std::vector<uint32_t> region(n, 0), stamp(n, 0);   /* region id of each vertex at this level */
-    std::vector<uint32_t> order(n), next_order;
+    std::vector<uint32_t> order(n), next_order(n), new_region(n);
     for (uint32_t v = 0; v < n; ++v) order[v] = v;
-    uint32_t nreg = 1, tick = 0;
-    std::vector<uint32_t> q;
+    /* regions are runs of 'order'; rstart[r] = first position of region r */
+    std::vector<uint64_t> rstart = {0, n};
+    uint32_t tick = 0;
     for (uint32_t size : SIZES) {
-        /* split every region (a run of 'order') into BFS blobs of 'size' */
-        next_order.clear();
-        next_order.reserve(n);
-        std::vector<uint32_t> new_region(n);
-        uint32_t new_nreg = 0;
+        /* split every region into BFS blobs of 'size' (seeds in the region's
+         * order); regions are disjoint, so they split in parallel, each
+         * writing its own run of next_order and local blob numbers */
         ++tick;   /* stamp == tick: already placed at this level */
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t s = order[i];
-            if (stamp[s] == tick) continue;
-            const uint32_t r = region[s];
-            q.clear();
-            q.push_back(s);
-            stamp[s] = tick;
-            for (size_t h = 0; h < q.size() && q.size() < size; ++h) {
-                const uint32_t u = q[h];
-                for (int dir = 0; dir < 2 && q.size() < size; ++dir) {
-                    const std::vector<uint32_t> &ptr = dir ? g->rrp : g->rp;
-                    const std::vector<uint32_t> &nb = dir ? g->rsrc : g->col;
-                    for (uint32_t e = ptr[u]; e < ptr[u + 1] && q.size() < size; ++e) {
-                        const uint32_t w = nb[e];
-                        if (stamp[w] == tick || region[w] != r) continue;
-                        stamp[w] = tick;
-                        q.push_back(w);
+        const uint32_t nreg = (uint32_t)rstart.size() - 1;
+        std::vector<uint32_t> nblob(nreg, 0);
+#pragma omp parallel
+        {
+            std::vector<uint32_t> q;
+#pragma omp for schedule(dynamic, 16)
+            for (uint32_t r = 0; r < nreg; ++r) {
+                uint64_t out = rstart[r];
+                uint32_t b = 0;
+                for (uint64_t i = rstart[r]; i < rstart[r + 1]; ++i) {
+                    const uint32_t s = order[i];
+                    if (stamp[s] == tick) continue;
+                    q.clear();
+                    q.push_back(s);
+                    stamp[s] = tick;
+                    for (size_t h = 0; h < q.size() && q.size() < size; ++h) {
+                        const uint32_t u = q[h];
+                        for (int dir = 0; dir < 2 && q.size() < size; ++dir) {
+                            const std::vector<uint32_t> &ptr = dir ? g->rrp : g->rp;
+                            const std::vector<uint32_t> &nb = dir ? g->rsrc : g->col;
+                            for (uint32_t e = ptr[u]; e < ptr[u + 1] && q.size() < size; ++e) {
+                                const uint32_t w = nb[e];
+                                if (region[w] != r || stamp[w] == tick) continue;
+                                stamp[w] = tick;
+                                q.push_back(w);
+                            }
+                        }
                     }
+                    for (uint32_t w : q) {
+                        new_region[w] = b;
+                        next_order[out++] = w;
+                    }
+                    ++b;
                 }
+                nblob[r] = b;
             }
-            for (uint32_t w : q) {
-                new_region[w] = new_nreg;
-                next_order.push_back(w);
-            }
-            ++new_nreg;
         }
+        /* global blob numbers: each region's blobs after those of the
+         * regions before it */
+        std::vector<uint64_t> boff((size_t)nreg + 1, 0);
+        for (uint32_t r = 0; r < nreg; ++r) boff[r + 1] = boff[r] + nblob[r];
+        std::vector<uint64_t> next_start((size_t)boff[nreg] + 1, 0);
+#pragma omp parallel for schedule(static, 4096)
+        for (uint32_t i = 0; i < n; ++i) new_region[next_order[i]] += (uint32_t)boff[region[next_order[i]]];
+        for (uint32_t i = 0; i < n; ++i) next_start[new_region[next_order[i]] + 1]++;
+        for (uint64_t b = 0; b < boff[nreg]; ++b) next_start[b + 1] += next_start[b];
         order.swap(next_order);
         region.swap(new_region);
-        nreg = new_nreg;
+        rstart.swap(next_start);
     }
-    (void)nreg;
     perm.resize(n);
+#pragma omp parallel for schedule(static, 4096)
     for (uint32_t i = 0; i < n; ++i) perm[order[i]] = i;
 }
 
@@ -497,12 +516,14 @@ static int ensure_blocked(fbc_ctx *ctx, fbc_graph *g)
     try {
         blocked_order(g, g->perm);
         std::vector<uint32_t> inv(n);
+#pragma omp parallel for schedule(static, 4096)
         for (uint32_t v = 0; v < n; ++v) inv[g->perm[v]] = v;
         auto build = [&](const std::vector<uint32_t> &ptr, const std::vector<uint32_t> &nb,
                          const std::vector<uint32_t> &wt, std::vector<uint32_t> &bp, std::vector<uint2> &arcs) {
             bp.assign((size_t)n + 1, 0);
             arcs.resize(m);
             for (uint32_t nu = 0; nu < n; ++nu) bp[nu + 1] = bp[nu] + (ptr[inv[nu] + 1] - ptr[inv[nu]]);
+#pragma omp parallel for schedule(static, 1024)
             for (uint32_t nu = 0; nu < n; ++nu) {
                 const uint32_t u = inv[nu];
                 uint2 *row = &arcs[bp[nu]];
