@@ -33,12 +33,15 @@
 namespace fbcdev {
 
 constexpr int HEAVY_DEG = 96;   /* out-degree handled by the whole workgroup */
-/* Tight-arc masks: bit j of sd[2v+1] (between the in-degree sweep and v's
- * dependency) says whether out-arc j of v lies on a shortest path, for
+/* Between the in-degree sweep and v's dependency, sd[2v+1] holds v's count
+ * of unprocessed DAG predecessors (high 32 bits) and its tight-arc mask (low
+ * 32 bits): bit j says whether out-arc j of v lies on a shortest path, for
  * out-degrees up to MASK_ARCS; MASK_NONE marks wider rows, whose arcs are
- * tested against the distances. */
-constexpr uint32_t MASK_ARCS = 63;
-constexpr uint64_t MASK_NONE = ~uint64_t(0);
+ * tested against the distances.  A Kahn step thus updates a successor's
+ * sigma and count in one 16-byte line segment. */
+constexpr uint32_t MASK_ARCS = 31;
+constexpr uint32_t MASK_NONE = 0xFFFFFFFFu;
+constexpr uint64_t PRED_ONE = 1ull << 32;   /* one predecessor in sd[2v+1] */
 constexpr int HEAVY_CAP = 64;
 /* Per team: RB arcs of one vertex have their loads and atomics issued
  * together; the ordered fold keeps up to SUCC successors in registers.  The
@@ -98,12 +101,11 @@ template <int SC> __device__ __forceinline__ uint64_t amax64(uint64_t *p, uint64
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
 /* Per-source state.  key[v] packs the tentative distance (high 32 bits) with
- * a tag (low 32 bits): during the distance phase the round in which v was
- * last queued, afterwards its count of unprocessed DAG predecessors.  sd[2v]
- * is sigma(v); sd[2v+1] holds v's tight-arc mask from the in-degree sweep
- * until the dependency phase replaces it by delta(v) (a double, or a float
- * pair, bit-cast), so a DAG successor's (sigma, delta) share one 16-byte line
- * segment. */
+ * a tag (low 32 bits): the round in which v was last queued.  sd[2v] is
+ * sigma(v); sd[2v+1] holds v's predecessor count and tight-arc mask from the
+ * in-degree sweep until the dependency phase replaces it by delta(v) (a
+ * double, or a float pair, bit-cast), so a DAG successor's (sigma, delta)
+ * share one 16-byte line segment. */
 struct Slot {
     uint64_t *key;
     uint32_t *fstamp, *qa, *qb, *fa, *fb;
@@ -557,17 +559,17 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
  * list. */
 template <class T>
 __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint64_t su, uint32_t e0,
-                                                  uint64_t mask, uint32_t *tail, uint32_t *err)
+                                                  uint32_t mask, uint32_t *tail, uint32_t *err)
 {
     constexpr int SC = T::SC, RB = T::RB;
-    uint64_t m = mask;
+    uint32_t m = mask;
     while (m) {
         uint32_t j[RB];
         bool ok[RB];
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             ok[k] = m != 0;
-            j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : 0u;
+            j[k] = ok[k] ? (uint32_t)__builtin_ctz(m) : 0u;
             m &= m - 1;
         }
         uint32_t w[RB];
@@ -577,13 +579,13 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             old[k] = ok[k] ? aadd64<SC>(&S.sd[2 * (uint64_t)w[k]], su) : 0;
-            cnt[k] = ok[k] ? asub64<SC>(&S.key[w[k]], 1) : 0;
+            cnt[k] = ok[k] ? asub64<SC>(&S.sd[2 * (uint64_t)w[k] + 1], PRED_ONE) : 0;
         }
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             if (!ok[k]) continue;
             if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
-            if ((uint32_t)cnt[k] == 1u) S.qa[aadd32<SC>(tail, 1u)] = w[k];
+            if ((cnt[k] >> 32) == 1u) S.qa[aadd32<SC>(tail, 1u)] = w[k];
         }
     }
 }
@@ -610,24 +612,23 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
             old[j] = tight[j] ? aadd64<SC>(&S.sd[2 * (uint64_t)a[j].x], su) : 0;
-            cnt[j] = tight[j] ? asub64<SC>(&S.key[a[j].x], 1) : 0;
+            cnt[j] = tight[j] ? asub64<SC>(&S.sd[2 * (uint64_t)a[j].x + 1], PRED_ONE) : 0;
         }
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
             if (!tight[j]) continue;
             if (old[j] + su < old[j]) atomicOr(err, ERR_SIGMA);
-            if ((uint32_t)cnt[j] == 1u) S.qa[aadd32<SC>(tail, 1u)] = a[j].x;
+            if ((cnt[j] >> 32) == 1u) S.qa[aadd32<SC>(tail, 1u)] = a[j].x;
         }
     }
 }
 
 /* Phase 2: DAG in-degrees and tight-arc masks, then the Kahn sweep with
  * exact sigma.  Returns the number of levels; the level list is S.qa, the
- * level starts S.fa.  The in-degree goes into the low half of key[v] (the
- * round tags are done with); the high half keeps the distance.  The sweep
- * walks the vertices in order, so the distance tests it makes for the masks
- * read neighbouring lines; the Kahn sweep and the dependency phase, which
- * visit vertices scattered, then touch only the successors. */
+ * level starts S.fa.  The in-degree and the mask go into sd[2v+1].  The
+ * sweep walks the vertices in order, so the distance tests it makes read
+ * neighbouring lines; the Kahn sweep and the dependency phase, which visit
+ * vertices scattered, then touch only the successors' sigma lines. */
 template <class T>
 __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t sort_cap, uint32_t *err, Counters &ct)
@@ -642,7 +643,7 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
         if (dv != DINF) {
             const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
             te += e1 - e0;
-            uint64_t mask = MASK_NONE;
+            uint32_t mask = MASK_NONE;
             if (e1 - e0 <= MASK_ARCS) {
                 mask = 0;
                 for (uint32_t eb = e0; eb < e1; eb += RB) {
@@ -654,10 +655,9 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
                     for (int j = 0; j < RB; ++j) dw[j] = eb + j < e1 ? dist_of<SC>(S, a[j].x) : DINF;
 #pragma unroll
                     for (int j = 0; j < RB; ++j)
-                        if (dw[j] != DINF && (uint64_t)dv + a[j].y == dw[j]) mask |= 1ull << (eb + j - e0);
+                        if (dw[j] != DINF && (uint64_t)dv + a[j].y == dw[j]) mask |= 1u << (eb + j - e0);
                 }
             }
-            S.sd[2 * (uint64_t)v + 1] = mask;
             if (v != src) {
                 const uint32_t r1 = V.rrp[v + 1];
                 for (uint32_t rb = V.rrp[v]; rb < r1; rb += RB) {
@@ -671,8 +671,8 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
                     for (int j = 0; j < RB; ++j) cin += du[j] != DINF && (uint64_t)du[j] + a[j].y == dv;
                 }
             }
+            S.sd[2 * (uint64_t)v + 1] = ((uint64_t)cin << 32) | mask;
         }
-        ast64<SC>(&S.key[v], ((uint64_t)dv << 32) | cin);
     }
     Local &L = *t.loc;
     if (te) atomicAdd(&L.te, te);
@@ -701,7 +701,7 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
                     continue;
                 }
             }
-            const uint64_t mask = ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
+            const uint32_t mask = (uint32_t)ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
             if (mask != MASK_NONE)
                 push_sigma_masked<T>(V, S, sigma_of<SC>(S, u), e0, mask, &c.tail, err);
             else
@@ -782,19 +782,20 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
     Acc<PREC> acc;
     const uint32_t dv = dist_final(S, v);
     const ulonglong2 own = sd_pair(S, v);
-    const uint64_t sv_int = own.x, mask = own.y;
+    const uint64_t sv_int = own.x;
+    const uint32_t mask = (uint32_t)own.y;   /* its predecessor count is 0 here */
     const double sv = (double)sv_int;
     const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
     if (!ORDERED && mask != MASK_NONE) {
         /* ascending arc order, RB successors' loads in flight together */
-        uint64_t m = mask;
+        uint32_t m = mask;
         while (m) {
             uint32_t j[RB];
             bool ok[RB];
 #pragma unroll
             for (int k = 0; k < RB; ++k) {
                 ok[k] = m != 0;
-                j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : 0u;
+                j[k] = ok[k] ? (uint32_t)__builtin_ctz(m) : 0u;
                 m &= m - 1;
             }
             uint32_t w[RB];
