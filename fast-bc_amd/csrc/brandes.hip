@@ -225,7 +225,7 @@ struct Counters {
 struct Ctl {
     uint32_t qcnt[3];     /* near-queue lengths by round mod 3 */
     uint32_t fcnt[2];     /* far-list lengths by window parity */
-    uint32_t minv[2];     /* smallest far distance by window parity */
+    uint32_t minv[3];     /* smallest far distance by window mod 3 */
     uint32_t tail;        /* level-list length */
 };
 
@@ -357,7 +357,7 @@ template <class T>
 __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t du, uint32_t e,
                                           uint32_t thr, uint32_t nxt, uint32_t win, uint32_t *qnext,
                                           uint32_t *qn_cnt, uint32_t *far, uint32_t *far_cnt,
-                                          uint32_t *err, Counters &ct)
+                                          uint32_t *far_min, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
     ++ct.c[STAT_RELAX];
@@ -378,9 +378,12 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
     if (nd < thr) {
         /* queued already in this round (with a larger distance): not again */
         if ((uint32_t)old != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = a.x;
-    } else if (amax32<SC>(&S.fstamp[a.x], win) < win) {
-        far[aadd32<SC>(far_cnt, 1u)] = a.x;
-        ++ct.c[STAT_FARPUSH];
+    } else {
+        amin32<SC>(far_min, nd);
+        if (amax32<SC>(&S.fstamp[a.x], win) < win) {
+            far[aadd32<SC>(far_cnt, 1u)] = a.x;
+            ++ct.c[STAT_FARPUSH];
+        }
     }
 }
 
@@ -391,7 +394,8 @@ template <class T>
 __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32_t du, uint32_t e0,
                                              uint32_t e1, uint32_t thr, uint32_t nxt, uint32_t win,
                                              uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far,
-                                             uint32_t *far_cnt, uint32_t *err, Counters &ct)
+                                             uint32_t *far_cnt, uint32_t *far_min, uint32_t *err,
+                                             Counters &ct)
 {
     constexpr int SC = T::SC, RB = T::RB;
     ct.c[STAT_RELAX] += e1 - e0;
@@ -426,9 +430,12 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
             ++ct.c[STAT_IMPROVE];
             if (nd < thr) {
                 if ((uint32_t)cur[j] != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = to[j];
-            } else if (amax32<SC>(&S.fstamp[to[j]], win) < win) {
-                far[aadd32<SC>(far_cnt, 1u)] = to[j];
-                ++ct.c[STAT_FARPUSH];
+            } else {
+                amin32<SC>(far_min, nd);
+                if (amax32<SC>(&S.fstamp[to[j]], win) < win) {
+                    far[aadd32<SC>(far_cnt, 1u)] = to[j];
+                    ++ct.c[STAT_FARPUSH];
+                }
             }
         }
     }
@@ -460,6 +467,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
         ast32<SC>(&c.fcnt[1], 0u);
         ast32<SC>(&c.minv[0], DINF);
         ast32<SC>(&c.minv[1], DINF);
+        ast32<SC>(&c.minv[2], DINF);
         S.qa[0] = src;
     }
     Local &L = *t.loc;
@@ -474,6 +482,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             const uint32_t nxt = rnd + 1;
             uint32_t *qn_cnt = &c.qcnt[nxt % 3];
             uint32_t *far_cnt = &c.fcnt[win & 1];
+            uint32_t *far_min = &c.minv[win % 3];
             uint32_t *nh_cnt = &L.nheavy[rnd & 1];
             if (t.leader()) {
                 ++ct.c[STAT_ROUNDS];
@@ -494,7 +503,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                         continue;
                     }
                 }
-                relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
+                relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
             }
             t.lsync();
             const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
@@ -503,7 +512,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                 const uint32_t du = dist_of<SC>(S, u);
                 const uint32_t e1 = V.rp[u + 1];
                 for (uint32_t e = V.rp[u] + t.lrank(); e < e1; e += t.lsize())
-                    relax_arc<T>(V, S, du, e, thr, nxt, win, qb, qn_cnt, fa, far_cnt, err, ct);
+                    relax_arc<T>(V, S, du, e, thr, nxt, win, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
             }
             t.sync();
             near = ald32<SC>(qn_cnt);
@@ -512,37 +521,34 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             if (t.aborted()) return;
         }
         /* window [.., thr) is settled; move on to the nearest far vertex.  The
-         * near queue of round rnd is empty and its counter is 0. */
+         * near queue of round rnd is empty and its counter is 0.  minv[win % 3]
+         * is the smallest distance any far relaxation (or carried entry) of
+         * this window set: a lower bound of the far distances (an entry may
+         * since have become near), and at least thr, so the next window
+         * [thr, m + delta) is never empty of progress.  Far entries are
+         * unique within a window (fstamp), so each is queued at most once and
+         * needs no round tag. */
         const uint32_t nfar = ald32<SC>(&c.fcnt[win & 1]);
         if (nfar == 0) break;
+        const uint32_t m = ald32<SC>(&c.minv[win % 3]);
         if (t.leader()) {
             ++ct.c[STAT_WINDOWS];
             ast32<SC>(&c.fcnt[(win + 1) & 1], 0u);   /* last read in the previous window */
-            ast32<SC>(&c.minv[(win + 1) & 1], DINF);
+            ast32<SC>(&c.minv[(win + 2) % 3], DINF); /* last read in the previous window */
         }
-        uint32_t mn = DINF;
-        for (uint32_t i = r0; i < nfar; i += R) {
-            const uint32_t d = dist_of<SC>(S, fa[i]);
-            if (d >= thr && d < mn) mn = d;
-        }
-        if (mn != DINF) amin32<SC>(&c.minv[win & 1], mn);
-        t.sync();
-        const uint32_t m = ald32<SC>(&c.minv[win & 1]);
         if (m == DINF) break;
         const uint64_t nthr64 = (uint64_t)m + delta;
         const uint32_t nthr = nthr64 >= DINF ? DINF : (uint32_t)nthr64;
         const uint32_t nwin = win + 1;
         for (uint32_t i = r0; i < nfar; i += R) {
             const uint32_t w = fa[i];
-            const uint32_t d = dist_of<SC>(S, w);
-            if (d < thr) continue;            /* settled in an earlier window */
+            const uint32_t d = dist_of<SC>(S, w);   /* no relaxation runs during the split */
+            if (d < thr) continue;            /* settled in this window */
             if (d < nthr) {
-                /* far entries carry tag 0; tag the round they are queued for
-                 * (no relaxation runs during the split, so d is stable) */
-                if ((uint32_t)amax64<SC>(&S.key[w], ((uint64_t)d << 32) | rnd) < rnd)
-                    qa[aadd32<SC>(&c.qcnt[rnd % 3], 1u)] = w;
-            } else if (amax32<SC>(&S.fstamp[w], nwin) < nwin) {
-                fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
+                qa[aadd32<SC>(&c.qcnt[rnd % 3], 1u)] = w;
+            } else {
+                amin32<SC>(&c.minv[nwin % 3], d);
+                if (amax32<SC>(&S.fstamp[w], nwin) < nwin) fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
             }
         }
         t.sync();
