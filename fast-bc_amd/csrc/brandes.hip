@@ -286,7 +286,7 @@ __device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
 /* One workgroup per source (batched kernel). */
 struct TeamWG {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 8, SUCC = 8;
+    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
     Ctl *ctl;
     Local *loc;
     uint32_t *sortbuf;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void wave_sync()
  * part of one workgroup). */
 struct TeamWave {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 4, SUCC = 4;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
     Ctl *ctl;
     Local *loc;
     __device__ uint32_t rank() const { return threadIdx.x & 63; }
@@ -334,7 +334,7 @@ struct TeamWave {
  * resident, so the grid barrier always completes). */
 struct TeamGrid {
     static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
-    static constexpr int RB = 8, SUCC = 8;
+    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
     Ctl *ctl;
     Local *loc;
     __device__ uint32_t rank() const { return blockIdx.x * WG + threadIdx.x; }
@@ -639,45 +639,74 @@ template <class T>
 __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
                                 uint32_t sort_cap, uint32_t *err, Counters &ct)
 {
-    constexpr int SC = T::SC, RB = T::RB;
+    constexpr int SC = T::SC;
     const uint32_t r0 = t.rank(), R = t.size();
     Ctl &c = *t.ctl;
     unsigned long long te = 0;
-    for (uint32_t v = r0; v < V.n; v += R) {
-        const uint32_t dv = dist_of<SC>(S, v);
-        uint32_t cin = 0;
-        if (dv != DINF) {
-            const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
-            te += e1 - e0;
-            uint32_t mask = MASK_NONE;
-            if (e1 - e0 <= MASK_ARCS) {
-                mask = 0;
-                for (uint32_t eb = e0; eb < e1; eb += RB) {
-                    uint2 a[RB];
+    /* SU vertices per thread and step, their out- and in-arcs SB at a time:
+     * every load of a step is independent of the others, so they are in
+     * flight together */
+    constexpr int SU = T::SWEEP_U, SB = 4;
+    for (uint32_t v0 = r0; v0 < V.n; v0 += SU * R) {
+        uint32_t dv[SU], e0[SU], e1[SU], f0[SU], f1[SU], cin[SU], mask[SU];
 #pragma unroll
-                    for (int j = 0; j < RB; ++j) a[j] = eb + j < e1 ? V.arcs[eb + j] : make_uint2(0u, 0u);
-                    uint32_t dw[RB];
+        for (int k = 0; k < SU; ++k) {
+            const uint32_t v = v0 + k * R;
+            const bool in = v < V.n;
+            dv[k] = in ? dist_of<SC>(S, v) : DINF;
+            e0[k] = in ? V.rp[v] : 0u;
+            e1[k] = in ? V.rp[v + 1] : 0u;
+            f0[k] = in ? V.rrp[v] : 0u;
+            f1[k] = in ? V.rrp[v + 1] : 0u;
+            cin[k] = 0;
+            mask[k] = 0;
+        }
 #pragma unroll
-                    for (int j = 0; j < RB; ++j) dw[j] = eb + j < e1 ? dist_of<SC>(S, a[j].x) : DINF;
-#pragma unroll
-                    for (int j = 0; j < RB; ++j)
-                        if (dw[j] != DINF && (uint64_t)dv + a[j].y == dw[j]) mask |= 1u << (eb + j - e0);
-                }
+        for (int k = 0; k < SU; ++k) {
+            if (dv[k] == DINF) {
+                e1[k] = e0[k];   /* unreached: nothing to test */
+                f1[k] = f0[k];
+            } else {
+                te += e1[k] - e0[k];
+                if (v0 + k * R == src) f1[k] = f0[k];   /* the source has no predecessors */
             }
-            if (v != src) {
-                const uint32_t r1 = V.rrp[v + 1];
-                for (uint32_t rb = V.rrp[v]; rb < r1; rb += RB) {
-                    uint2 a[RB];
+        }
+        for (uint32_t b = 0;; b += SB) {
+            bool any = false;
 #pragma unroll
-                    for (int j = 0; j < RB; ++j) a[j] = rb + j < r1 ? V.rarcs[rb + j] : make_uint2(0u, 0u);
-                    uint32_t du[RB];
+            for (int k = 0; k < SU; ++k) any = any || e0[k] + b < e1[k] || f0[k] + b < f1[k];
+            if (!any) break;
+            uint2 ao[SU][SB], ai[SU][SB];
 #pragma unroll
-                    for (int j = 0; j < RB; ++j) du[j] = rb + j < r1 ? dist_of<SC>(S, a[j].x) : DINF;
+            for (int k = 0; k < SU; ++k)
 #pragma unroll
-                    for (int j = 0; j < RB; ++j) cin += du[j] != DINF && (uint64_t)du[j] + a[j].y == dv;
+                for (int j = 0; j < SB; ++j) {
+                    ao[k][j] = e0[k] + b + j < e1[k] ? V.arcs[e0[k] + b + j] : make_uint2(0u, 0u);
+                    ai[k][j] = f0[k] + b + j < f1[k] ? V.rarcs[f0[k] + b + j] : make_uint2(0u, 0u);
                 }
-            }
-            S.sd[2 * (uint64_t)v + 1] = ((uint64_t)cin << 32) | mask;
+            uint32_t dout[SU][SB], din[SU][SB];
+#pragma unroll
+            for (int k = 0; k < SU; ++k)
+#pragma unroll
+                for (int j = 0; j < SB; ++j) {
+                    dout[k][j] = e0[k] + b + j < e1[k] ? dist_of<SC>(S, ao[k][j].x) : DINF;
+                    din[k][j] = f0[k] + b + j < f1[k] ? dist_of<SC>(S, ai[k][j].x) : DINF;
+                }
+#pragma unroll
+            for (int k = 0; k < SU; ++k)
+#pragma unroll
+                for (int j = 0; j < SB; ++j) {
+                    if (dout[k][j] != DINF && (uint64_t)dv[k] + ao[k][j].y == dout[k][j] && b + j < 32)
+                        mask[k] |= 1u << (b + j);
+                    cin[k] += din[k][j] != DINF && (uint64_t)din[k][j] + ai[k][j].y == dv[k];
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const uint32_t v = v0 + k * R;
+            if (v >= V.n || dv[k] == DINF) continue;
+            const uint32_t mk = e1[k] - e0[k] > MASK_ARCS ? MASK_NONE : mask[k];
+            S.sd[2 * (uint64_t)v + 1] = ((uint64_t)cin[k] << 32) | mk;
         }
     }
     Local &L = *t.loc;
@@ -931,19 +960,33 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
         if (t.aborted()) return;
     }
     /* the source and unreached vertices have no dependency (their delta
-     * slots hold older values) */
-    for (uint32_t v = r0; v < V.n; v += R) {
-        const bool dep = v != src && dist_final(S, v) != DINF;
-        const double dv = dep ? Acc<PREC>::value_of(S.sd[2 * (uint64_t)v + 1]) : 0.0;
-        if (MODE == MODE_DEP) {
-            p.out_dep[(uint64_t)task * V.n + v] = dv;
-        } else if (MODE == MODE_EXACT) {
-            if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
-        } else if (MODE == MODE_PIVOT) {
-            const double x = dv * wgt;
-            if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
-        } else if (MODE == MODE_CLUSTER) {
-            p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
+     * slots hold older values); OU vertices per thread have their loads in
+     * flight together */
+    constexpr int OU = 4;
+    for (uint32_t v0 = r0; v0 < V.n; v0 += OU * R) {
+        uint32_t d[OU];
+        uint64_t b[OU];
+#pragma unroll
+        for (int k = 0; k < OU; ++k) {
+            const uint32_t v = v0 + k * R;
+            d[k] = v < V.n ? dist_final(S, v) : DINF;
+            b[k] = v < V.n ? S.sd[2 * (uint64_t)v + 1] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < OU; ++k) {
+            const uint32_t v = v0 + k * R;
+            if (v >= V.n) break;
+            const double dv = v != src && d[k] != DINF ? Acc<PREC>::value_of(b[k]) : 0.0;
+            if (MODE == MODE_DEP) {
+                p.out_dep[(uint64_t)task * V.n + v] = dv;
+            } else if (MODE == MODE_EXACT) {
+                if (dv != 0.0) unsafeAtomicAdd(&p.bc[v], dv);
+            } else if (MODE == MODE_PIVOT) {
+                const double x = dv * wgt;
+                if (x != 0.0) unsafeAtomicAdd(&p.bc[v], x);
+            } else if (MODE == MODE_CLUSTER) {
+                p.dmat[p.cl_doff[c] + (uint64_t)src * V.n + v] = dv;
+            }
         }
     }
 }
