@@ -845,28 +845,56 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         }
         return acc;
     }
-    if (ORDERED && mask != MASK_NONE && e1 - e0 <= (uint32_t)SUCC_REG) {
-        uint2 a[SUCC_REG];
+    if (ORDERED && mask != MASK_NONE) {
+        if (__popc(mask) <= SUCC_REG) {
+            /* the successors' (sigma, delta) loads go out together; only the
+             * sums follow the order */
+            uint64_t key[SUCC_REG];
+            uint32_t m = mask;
 #pragma unroll
-        for (int j = 0; j < SUCC_REG; ++j)
-            a[j] = (mask >> j) & 1 ? V.arcs[e0 + j] : make_uint2(0u, 0u);
-        uint64_t key[SUCC_REG];
+            for (int j = 0; j < SUCC_REG; ++j) {
+                key[j] = 0;
+                if (m) {
+                    const uint2 a = V.arcs[e0 + __builtin_ctz(m)];
+                    key[j] = ((uint64_t)(dv + a.y) << 32) | a.x;
+                    m &= m - 1;
+                }
+            }
+            ulonglong2 sw[SUCC_REG];
 #pragma unroll
-        for (int j = 0; j < SUCC_REG; ++j)
-            key[j] = (mask >> j) & 1 ? ((uint64_t)(dv + a[j].y) << 32) | a[j].x : 0;
+            for (int j = 0; j < SUCC_REG; ++j) sw[j] = key[j] ? sd_pair(S, (uint32_t)key[j]) : make_ulonglong2(0, 0);
 #pragma unroll
-        for (int r = 0; r < SUCC_REG; ++r) {
+            for (int r = 0; r < SUCC_REG; ++r) {
+                uint64_t best = 0;
+                int bj = -1;
+#pragma unroll
+                for (int j = 0; j < SUCC_REG; ++j)
+                    if (key[j] > best) { best = key[j]; bj = j; }
+                if (bj < 0) break;
+                ulonglong2 b = make_ulonglong2(0, 0);
+#pragma unroll
+                for (int j = 0; j < SUCC_REG; ++j)
+                    if (j == bj) {
+                        b = sw[j];
+                        key[j] = 0;
+                    }
+                acc.add(sv_int, sv, b.x, b.y);
+            }
+            return acc;
+        }
+        /* more successors than registers: repeated selection over the mask */
+        uint64_t prev = ~uint64_t(0);
+        for (;;) {
             uint64_t best = 0;
-            int bj = -1;
-#pragma unroll
-            for (int j = 0; j < SUCC_REG; ++j)
-                if (key[j] > best) { best = key[j]; bj = j; }
-            if (bj < 0) break;
-#pragma unroll
-            for (int j = 0; j < SUCC_REG; ++j)
-                if (j == bj) key[j] = 0;
-            const ulonglong2 b = sd_pair(S, (uint32_t)best);
-            acc.add(sv_int, sv, b.x, b.y);
+            for (uint32_t m = mask; m; m &= m - 1) {
+                const uint2 a = V.arcs[e0 + __builtin_ctz(m)];
+                const uint64_t k = ((uint64_t)(dv + a.y) << 32) | a.x;
+                if (k < prev && k > best) best = k;
+            }
+            if (best == 0) break;
+            const ulonglong2 sw = sd_pair(S, (uint32_t)best);
+            acc.add(sv_int, sv, sw.x, sw.y);
+            prev = best;
         }
         return acc;
     }
