@@ -153,16 +153,16 @@ uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
 int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out)
 {
     const uint64_t sb = fbcdev::slot_bytes(nmax);
-    /* 8 workgroups per CU; a wavefront-per-source workgroup holds 4 slots */
-    const uint32_t per_cu = fbcdev::wave_team(nmax) ? 32u : 8u;
+    /* whole-graph slots: 4 per CU (throughput levels off near 3); a
+     * wavefront-per-source workgroup holds 4 slots, 8 such workgroups per CU */
+    const uint32_t per_cu = fbcdev::wave_team(nmax) ? 32u : 4u;
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
-    /* whole-graph slots (tens of MB each) are bounded by memory: throughput
-     * levels off near 3 per CU (about 40 GiB of 1M-vertex slots), where the
-     * memory system, not occupancy, limits the launch */
-    const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20
-                                          : std::min<size_t>(ctx->total_mem / 4, (size_t)40 << 30);
+    /* whole-graph slots (44 B per vertex each) are also bounded by memory:
+     * a third of the device (4 per CU of a 1M-vertex graph take 45 GB; a
+     * 4M-vertex graph gets ~2 per CU) */
+    const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20 : ctx->total_mem / 3;
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     /* the wavefront kernel's workgroups address WAVES slots each: hold whole
      * workgroups' worth */

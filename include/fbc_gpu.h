@@ -75,8 +75,8 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * (1, the default: the batched pivot phase and fbc_exact_bc run on a copy of
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
  * locality; 0: on the graph as given), "scratch_mb" (device memory for the
- * sources in flight, in MiB; automatic: a quarter of the device, at most
- * 40 GiB), "class_key_bits" (bits of the row hashes fbc_cluster_classes
+ * sources in flight, in MiB; automatic: up to 4 whole-graph sources per CU
+ * within a third of the device), "class_key_bits" (bits of the row hashes fbc_cluster_classes
  * keeps; 0: all 64; a few bits force collisions, a test hook).  No
  * counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
