@@ -90,7 +90,16 @@ __device__ uint32_t block_scan(uint32_t x, uint32_t *s_wave, uint32_t *total)
     return base + inc - x;
 }
 
-/* grid: one workgroup per tile (s, ti, tj), tj >= ti */
+/* FP64 matrix-core block: D(16x16) += A(16x4) B(4x16).  Lane l supplies
+ * A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4r][l % 16]
+ * in element r (checked on gfx950 against a host product). */
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+/* grid: one workgroup per 64 x 64 tile (s, ti, tj), tj >= ti.  Wave w
+ * computes the 32 x 32 quarter (w / 2, w % 2) of the tile as 2 x 2 blocks of
+ * FP64 MFMA; the features stream through LDS GD at a time.  Every product
+ * and partial sum is an integer below 2^53 (the exactness precondition), so
+ * the matrix cores' summation order gives the exact Gram entries. */
 __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const uint32_t *tiles)
 {
     const uint32_t s = tiles[3 * blockIdx.x], ti = tiles[3 * blockIdx.x + 1], tj = tiles[3 * blockIdx.x + 2];
@@ -99,9 +108,15 @@ __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const ui
     double *Gs = G + p.goff[s];
     __shared__ double sa[GD][GT + 1];
     __shared__ double sb[GD][GT + 1];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int li = lane % 16, lk = lane / 16;
+    const int wr = (w / 2) * 32, wc = (w % 2) * 32;
     const uint32_t r0 = ti * GT, c0 = tj * GT;
-    double acc[4][4] = {};
+    dbl4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
     for (uint32_t d0 = 0; d0 < D; d0 += GD) {
         for (int e = threadIdx.x; e < GD * GT; e += GWG) {
             const int dd = e % GD, rr = e / GD;
@@ -111,31 +126,30 @@ __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const ui
         }
         __syncthreads();
 #pragma unroll
-        for (int dd = 0; dd < GD; ++dd) {
-            double xv[4], cv[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                xv[a] = sa[dd][ty * 4 + a];
-                cv[a] = sb[dd][tx * 4 + a];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fma(xv[a], cv[b], acc[a][b]);
+        for (int kk = 0; kk < GD; kk += 4) {
+            const double a0 = sa[kk + lk][wr + li], a1 = sa[kk + lk][wr + 16 + li];
+            const double b0 = sb[kk + lk][wc + li], b1 = sb[kk + lk][wc + 16 + li];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t r = r0 + ty * 4 + a, c = c0 + tx * 4 + b;
-            if (r < P && c < P) {
-                Gs[(uint64_t)r * P + c] = acc[a][b];
-                Gs[(uint64_t)c * P + r] = acc[a][b];
-                if (r == c) p.xnorm[p.poff[s] + r] = acc[a][b];
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t row = r0 + wr + a * 16 + lk + 4 * r, col = c0 + wc + b * 16 + li;
+                if (row < P && col < P) {
+                    const double v = acc[a][b][r];
+                    Gs[(uint64_t)row * P + col] = v;
+                    Gs[(uint64_t)col * P + row] = v;
+                    if (row == col) p.xnorm[p.poff[s] + row] = v;
+                }
             }
-        }
 }
 
 __global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
