@@ -30,7 +30,8 @@ EXPORTS = (
 )
 
 # Every symbol include/fbc_host.h declares.
-HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_exact_bc_arcs")
+HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_exact_bc_arcs",
+                "fbc_host_release")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
                  "ns_dist", "ns_sigma", "ns_dep")
@@ -110,6 +111,8 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p, i32p,
                                        _f64p, _u32p, _u32p, _u32p, _f64p]
         L.fbc_exact_bc_arcs.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, ctypes.c_int, _f64p]
+        L.fbc_host_release.argtypes = []
+        L.fbc_host_release.restype = None
         _lib = L
     return _lib
 
@@ -453,6 +456,11 @@ def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, d
                     "fbc_clustered_bc")
     return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=pw[:k], pivot_cluster=pc[:k],
                 phases=dict(zip(PHASES, ph.tolist())))
+
+
+def host_release():
+    """Release the device contexts and the resident graph the host engine keeps."""
+    lib().fbc_host_release()
 
 
 def exact_bc_arcs(src, dst, w, prec=PREC_F64, device=0):

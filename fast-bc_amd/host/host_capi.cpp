@@ -2,11 +2,16 @@
 #include "fbc_host.h"
 #include "../../include/fbc_host.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 using namespace fbchost;
 
@@ -28,9 +33,76 @@ int guarded(F f)
     }
 }
 
+/* What the engine keeps between calls: one device context per HIP device
+ * (with the FBC_OPTIONS it was configured with) and the last graph it built,
+ * with a copy of the arcs it was built from.  A call on the same arcs finds
+ * the graph, its device copy, its relabelled copy and the per-source scratch
+ * resident, as a caller holding a graph object would; other arcs replace it.
+ * Contexts are released by fbc_host_release, not at exit (the HIP runtime
+ * may be gone by then). */
+struct Resident {
+    std::mutex mu;
+    std::map<int, std::pair<std::string, Device *>> devices;
+    std::vector<int32_t> src, dst;
+    std::vector<double> w;
+    std::shared_ptr<Graph> graph;
+};
+
+Resident &resident()
+{
+    static Resident *r = new Resident;
+    return *r;
+}
+
+/* The device context for 'device', created (or re-created when FBC_OPTIONS
+ * changed) under the resident lock. */
+Device &device_for(int device)
+{
+    Resident &r = resident();
+    const char *env = std::getenv("FBC_OPTIONS");
+    const std::string opts = env ? env : "";
+    auto it = r.devices.find(device);
+    if (it != r.devices.end() && it->second.first == opts) return *it->second.second;
+    if (it != r.devices.end()) {
+        delete it->second.second;
+        r.devices.erase(it);
+    }
+    Device *d = new Device(device);
+    r.devices[device] = {opts, d};
+    return *d;
+}
+
+std::shared_ptr<Graph> graph_for(uint64_t m, const int32_t *src, const int32_t *dst, const double *w)
+{
+    Resident &r = resident();
+    if (r.graph && r.src.size() == m && std::equal(src, src + m, r.src.begin()) &&
+        std::equal(dst, dst + m, r.dst.begin()) &&
+        std::memcmp(w, r.w.data(), m * sizeof(double)) == 0)
+        return r.graph;
+    r.graph.reset();
+    auto g = std::make_shared<Graph>(from_arcs(m, src, dst, w));
+    r.src.assign(src, src + m);
+    r.dst.assign(dst, dst + m);
+    r.w.assign(w, w + m);
+    r.graph = g;
+    return g;
+}
+
 } // namespace
 
 extern "C" {
+
+void fbc_host_release(void)
+{
+    Resident &r = resident();
+    std::lock_guard<std::mutex> lk(r.mu);
+    for (auto &kv : r.devices) delete kv.second.second;
+    r.devices.clear();
+    r.graph.reset();
+    r.src = {};
+    r.dst = {};
+    r.w = {};
+}
 
 int fbc_louvain(uint64_t m, const int32_t *src, const int32_t *dst, const double *w,
                 const uint64_t *seeds, int nseeds, double precision, int32_t *comm_of)
@@ -88,8 +160,10 @@ int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *
     if (prec != FBC_PREC_F64 && prec != FBC_PREC_F32C) return FBC_E_ARG;
     if (form != FBC_FORM_BATCHED && form != FBC_FORM_PLAIN && form != FBC_FORM_SELECT_ONLY) return FBC_E_ARG;
     return guarded([&] {
-        Graph g = from_arcs(m, src, dst, w);
-        Device dev(device);
+        std::lock_guard<std::mutex> lk(resident().mu);
+        const std::shared_ptr<Graph> gp = graph_for(m, src, dst, w);
+        const Graph &g = *gp;
+        Device &dev = device_for(device);
         ClusteredOptions opt;
         opt.seeds.assign(seeds, seeds + nseeds);
         opt.precision = precision;
@@ -128,9 +202,9 @@ int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t 
     if ((m && (!src || !dst || !w)) || !bc) return FBC_E_ARG;
     if (prec != FBC_PREC_F64 && prec != FBC_PREC_F32C) return FBC_E_ARG;
     return guarded([&] {
-        Graph g = from_arcs(m, src, dst, w);
-        Device dev(device);
-        std::vector<double> r = exact_bc(dev, g, prec);
+        std::lock_guard<std::mutex> lk(resident().mu);
+        const std::shared_ptr<Graph> gp = graph_for(m, src, dst, w);
+        std::vector<double> r = exact_bc(device_for(device), *gp, prec);
         std::memcpy(bc, r.data(), r.size() * sizeof(double));
         return 0;
     });

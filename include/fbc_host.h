@@ -62,6 +62,13 @@ int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,
                       const double *w, int prec, double *bc);
 
+/* fbc_clustered_bc and fbc_exact_bc_arcs keep one device context per device
+ * (configured from FBC_OPTIONS, re-created when it changes) and the graph of
+ * the last arcs they were given, so that a repeated call on the same arcs
+ * finds the graph and its device copies resident.  This releases them (and
+ * the device memory they hold).  No counterpart in the reference. */
+void fbc_host_release(void);
+
 #ifdef __cplusplus
 }
 #endif
