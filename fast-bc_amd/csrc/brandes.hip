@@ -239,17 +239,25 @@ struct Local {
 };
 
 /* Per-workgroup LDS of the workgroup-per-source and grid kernels. */
+/* Near queues kept in LDS: the first LQ entries of a round's queue (two
+ * queues, this round's and the next), the rest in the slot's global queue.
+ * A round then reads its vertices from LDS instead of one more dependent
+ * global load. */
+constexpr uint32_t LQ_WG = 1024, LQ_WAVE = 256;
+
 struct Shared {
     Ctl ctl;
     Local loc;
     uint32_t task;
     uint32_t sortbuf[SORT_CAP_MAX];
+    uint32_t lq[2 * LQ_WG];
 };
 
 /* Per-workgroup LDS of the wavefront-per-source kernel. */
 struct SharedWave {
     Ctl ctl[WAVES];
     Local loc[WAVES];
+    uint32_t lq[WAVES][2 * LQ_WAVE];
 };
 
 /* Sort q[0, n) ascending through LDS (bitonic; n <= SORT_CAP_MAX), so that a
@@ -287,9 +295,11 @@ __device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
 struct TeamWG {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
     static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
+    static constexpr uint32_t LQ = LQ_WG;
     Ctl *ctl;
     Local *loc;
     uint32_t *sortbuf;
+    uint32_t *lq;
     __device__ uint32_t rank() const { return threadIdx.x; }
     __device__ uint32_t size() const { return WG; }
     __device__ bool leader() const { return threadIdx.x == 0; }
@@ -316,8 +326,10 @@ __device__ __forceinline__ void wave_sync()
 struct TeamWave {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
     static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
+    static constexpr uint32_t LQ = LQ_WAVE;
     Ctl *ctl;
     Local *loc;
+    uint32_t *lq;
     __device__ uint32_t rank() const { return threadIdx.x & 63; }
     __device__ uint32_t size() const { return 64; }
     __device__ bool leader() const { return (threadIdx.x & 63) == 0; }
@@ -335,8 +347,10 @@ struct TeamWave {
 struct TeamGrid {
     static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
     static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
+    static constexpr uint32_t LQ = 0;   /* the grid shares no LDS */
     Ctl *ctl;
     Local *loc;
+    uint32_t *lq;
     __device__ uint32_t rank() const { return blockIdx.x * WG + threadIdx.x; }
     __device__ uint32_t size() const { return gridDim.x * WG; }
     __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
@@ -351,12 +365,31 @@ struct TeamGrid {
 template <class T> struct IsWG { static constexpr bool value = false; };
 template <> struct IsWG<TeamWG> { static constexpr bool value = true; };
 
+/* Append w to a near queue whose first T::LQ entries live in LDS (lq) and
+ * the rest in the slot's global queue (q). */
+template <class T>
+__device__ __forceinline__ void push_near(uint32_t *lq, uint32_t *q, uint32_t *cnt, uint32_t w)
+{
+    const uint32_t i = aadd32<T::SC>(cnt, 1u);
+    if (T::LQ && i < T::LQ)
+        lq[i] = w;
+    else
+        q[i] = w;
+}
+
+/* Entry i of such a queue. */
+template <class T>
+__device__ __forceinline__ uint32_t near_entry(const uint32_t *lq, const uint32_t *q, uint32_t i)
+{
+    return T::LQ && i < T::LQ ? lq[i] : q[i];
+}
+
 /* Relax arc e of u (distance du); used for the arcs of heavy vertices, one
  * per thread. */
 template <class T>
 __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t du, uint32_t e,
-                                          uint32_t thr, uint32_t nxt, uint32_t win, uint32_t *qnext,
-                                          uint32_t *qn_cnt, uint32_t *far, uint32_t *far_cnt,
+                                          uint32_t thr, uint32_t nxt, uint32_t win, uint32_t *lnext,
+                                          uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far, uint32_t *far_cnt,
                                           uint32_t *far_min, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
@@ -377,7 +410,7 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
     ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
         /* queued already in this round (with a larger distance): not again */
-        if ((uint32_t)old != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = a.x;
+        if ((uint32_t)old != nxt) push_near<T>(lnext, qnext, qn_cnt, a.x);
     } else {
         amin32<SC>(far_min, nd);
         if (amax32<SC>(&S.fstamp[a.x], win) < win) {
@@ -393,7 +426,7 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
 template <class T>
 __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32_t du, uint32_t e0,
                                              uint32_t e1, uint32_t thr, uint32_t nxt, uint32_t win,
-                                             uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far,
+                                             uint32_t *lnext, uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far,
                                              uint32_t *far_cnt, uint32_t *far_min, uint32_t *err,
                                              Counters &ct)
 {
@@ -429,7 +462,7 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
             if ((cur[j] >> 32) <= nd) continue;
             ++ct.c[STAT_IMPROVE];
             if (nd < thr) {
-                if ((uint32_t)cur[j] != nxt) qnext[aadd32<SC>(qn_cnt, 1u)] = to[j];
+                if ((uint32_t)cur[j] != nxt) push_near<T>(lnext, qnext, qn_cnt, to[j]);
             } else {
                 amin32<SC>(far_min, nd);
                 if (amax32<SC>(&S.fstamp[to[j]], win) < win) {
@@ -468,13 +501,17 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
         ast32<SC>(&c.minv[0], DINF);
         ast32<SC>(&c.minv[1], DINF);
         ast32<SC>(&c.minv[2], DINF);
-        S.qa[0] = src;
+        if (T::LQ)
+            t.lq[0] = src;
+        else
+            S.qa[0] = src;
     }
     Local &L = *t.loc;
     if (t.lleader()) L.nheavy[0] = L.nheavy[1] = 0;
     t.sync();
 
     uint32_t *qa = S.qa, *qb = S.qb, *fa = S.fa, *fb = S.fb;
+    uint32_t *lqa = t.lq, *lqb = t.lq + T::LQ;   /* the queues' LDS heads */
     uint32_t near = 1, rnd = 1, win = 1;
     uint32_t thr = delta;
     for (;;) {
@@ -490,9 +527,9 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             }
             if (t.lleader()) L.nheavy[(rnd + 1) & 1] = 0;
             if constexpr (IsWG<T>::value)
-                if (near > 64 && near <= sort_cap) sort_vertices(qa, near, t.sortbuf);
+                if (near > 64 && near <= sort_cap) sort_vertices(near <= T::LQ ? lqa : qa, near, t.sortbuf);
             for (uint32_t i = r0; i < near; i += R) {
-                const uint32_t u = qa[i];
+                const uint32_t u = near_entry<T>(lqa, qa, i);
                 ++ct.c[STAT_VISITS];
                 const uint32_t du = dist_of<SC>(S, u);
                 const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
@@ -503,7 +540,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                         continue;
                     }
                 }
-                relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
+                relax_vertex<T>(V, S, du, e0, e1, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
             }
             t.lsync();
             const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
@@ -512,11 +549,12 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                 const uint32_t du = dist_of<SC>(S, u);
                 const uint32_t e1 = V.rp[u + 1];
                 for (uint32_t e = V.rp[u] + t.lrank(); e < e1; e += t.lsize())
-                    relax_arc<T>(V, S, du, e, thr, nxt, win, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
+                    relax_arc<T>(V, S, du, e, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
             }
             t.sync();
             near = ald32<SC>(qn_cnt);
             uint32_t *tq = qa; qa = qb; qb = tq;
+            uint32_t *tl = lqa; lqa = lqb; lqb = tl;
             rnd = nxt;
             if (t.aborted()) return;
         }
@@ -545,7 +583,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             const uint32_t d = dist_of<SC>(S, w);   /* no relaxation runs during the split */
             if (d < thr) continue;            /* settled in this window */
             if (d < nthr) {
-                qa[aadd32<SC>(&c.qcnt[rnd % 3], 1u)] = w;
+                push_near<T>(lqa, qa, &c.qcnt[rnd % 3], w);
             } else {
                 amin32<SC>(&c.minv[nwin % 3], d);
                 if (amax32<SC>(&S.fstamp[w], nwin) < nwin) fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
@@ -1098,7 +1136,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
     __shared__ Shared sh;
     Slot S;
     slot_layout(p.nmax, p.scratch + (uint64_t)blockIdx.x * p.slot_bytes, &S);
-    const TeamWG t{&sh.ctl, &sh.loc, sh.sortbuf};
+    const TeamWG t{&sh.ctl, &sh.loc, sh.sortbuf, sh.lq};
     if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;
     for (;;) {
@@ -1122,7 +1160,7 @@ __global__ __launch_bounds__(WG) void k_brandes_wave(BrandesParams p)
     if (blockIdx.x * WAVES + w >= p.nslots) return;   /* no slot behind this wavefront */
     Slot S;
     slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * WAVES + w) * p.slot_bytes, &S);
-    const TeamWave t{&sw.ctl[w], &sw.loc[w]};
+    const TeamWave t{&sw.ctl[w], &sw.loc[w], sw.lq[w]};
     if (t.lleader()) sw.loc[w].te = 0;
     Counters ct;
     for (;;) {
@@ -1143,7 +1181,7 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     __shared__ Shared sh;
     Slot S;
     slot_layout(p.nmax, p.scratch, &S);
-    const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl), &sh.loc};
+    const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl), &sh.loc, nullptr};
     if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;
     View V;
