@@ -172,6 +172,30 @@ T parse_value(const Opt &o, bool is_long)
 
 } // namespace
 
+/* bc.txt as main.cpp:282-294 writes it: one `out << x << std::endl` per
+ * vertex (default stream formatting, i.e. %g with 6 significant digits),
+ * negative values as 0.  The lines are formatted in parallel and written in
+ * order. */
+static void write_bc(const std::string &path, const std::vector<double> &bc)
+{
+    const size_t n = bc.size();
+    const int nchunk = n < 65536 ? 1 : 64;
+    std::vector<std::string> parts((size_t)nchunk);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < nchunk; ++k) {
+        const size_t a = n * (size_t)k / nchunk, b = n * (size_t)(k + 1) / nchunk;
+        std::string &s = parts[(size_t)k];
+        s.reserve((b - a) * 12);
+        char buf[64];
+        for (size_t i = a; i < b; ++i) {
+            const int len = bc[i] >= 0 ? std::snprintf(buf, sizeof buf, "%g\n", bc[i]) : std::snprintf(buf, sizeof buf, "0\n");
+            s.append(buf, (size_t)len);
+        }
+    }
+    std::ofstream out(path, std::ofstream::out);
+    for (const std::string &s : parts) out.write(s.data(), (std::streamsize)s.size());
+}
+
 int main(int argc, char **argv)
 {
     Parser op;
@@ -316,11 +340,7 @@ int main(int argc, char **argv)
     }
     log(INFO, "Total computation time: " + std::to_string(ms) + "." + std::to_string(us) + "ms");
 
-    std::ofstream out(outpath, std::ofstream::out);
-    for (size_t i = 0; i < bc.size(); ++i) {
-        if (bc[i] >= 0) out << bc[i] << std::endl;
-        else out << 0 << std::endl;
-    }
+    write_bc(outpath, bc);
     if (!fullout.empty()) {
         FILE *f = std::fopen(fullout.c_str(), "w");
         if (f) {

@@ -7,13 +7,17 @@
  *   - an unreadable id ends the input, as the reference's stream loop does;
  *   - repeated arcs are summed in file order and counted once;
  *   - the vertex count is 1 + the largest source id.
- * The text is parsed in one pass over a buffer instead of iostream
- * extraction; arcs are grouped by a stable sort instead of std::map inserts.
+ * The text is read whole and parsed in parallel chunks cut at line ends
+ * (merged in file order, with the sequential parse's stop and error rules;
+ * a triple split across a chunk boundary sends the file to the sequential
+ * parse); arcs are grouped by a counting sort on the source, stable, then by
+ * target within each row, instead of std::map inserts.
  */
 #include "fbc_host.h"
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cerrno>
 #include <climits>
 #include <cstdlib>
@@ -99,26 +103,57 @@ Graph from_arcs_n(int n, size_t m, const int *src, const int *dst, const double 
                                         " has no out-arc (the vertex count is 1 + the largest source id)");
     g.in_wdeg.assign((size_t)std::max(g.n, 0), 0.0);
     g.out_wdeg.assign((size_t)std::max(g.n, 0), 0.0);
-    /* file-order sums of repeated arcs (DirectedWeightedGraph.h:154-165) */
-    std::vector<size_t> order(m);
-    std::iota(order.begin(), order.end(), (size_t)0);
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        return src[a] != src[b] ? src[a] < src[b] : dst[a] < dst[b];
-    });
-    g.rp.assign((size_t)g.n + 1, 0);
-    g.col.reserve(m);
-    g.w.reserve(m);
-    for (size_t i = 0; i < m;) {
-        const size_t a = order[i];
-        double sum = w[a];
-        size_t j = i + 1;
-        while (j < m && src[order[j]] == src[a] && dst[order[j]] == dst[a]) sum += w[order[j++]];
-        g.col.push_back(dst[a]);
-        g.w.push_back(sum);
-        g.rp[(size_t)src[a] + 1]++;
-        i = j;
+    /* file-order sums of repeated arcs (DirectedWeightedGraph.h:154-165):
+     * arcs bucketed by source in file order, each bucket stably sorted by
+     * target, equal targets summed in file order */
+    const size_t nn = (size_t)std::max(g.n, 0);
+    std::vector<uint64_t> bstart(nn + 1, 0);
+    for (size_t i = 0; i < m; ++i) bstart[(size_t)src[i] + 1]++;
+    for (size_t v = 0; v < nn; ++v) bstart[v + 1] += bstart[v];
+    std::vector<uint32_t> idx(m);
+    {
+        std::vector<uint64_t> fill(bstart.begin(), bstart.end() - 1);
+        for (size_t i = 0; i < m; ++i) idx[fill[src[i]]++] = (uint32_t)i;
     }
-    for (int v = 0; v < g.n; ++v) g.rp[v + 1] += g.rp[v];
+    std::vector<uint32_t> ucount(nn, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (size_t v = 0; v < nn; ++v) {
+        uint32_t *b = idx.data() + bstart[v], *e = idx.data() + bstart[v + 1];
+        if (e - b > 32)
+            std::stable_sort(b, e, [&](uint32_t x, uint32_t y) { return dst[x] < dst[y]; });
+        else
+            for (uint32_t *p = b + 1; p < e; ++p) {   /* insertion sort, stable */
+                const uint32_t x = *p;
+                uint32_t *q = p;
+                while (q > b && dst[q[-1]] > dst[x]) {
+                    *q = q[-1];
+                    --q;
+                }
+                *q = x;
+            }
+        uint32_t u = 0;
+        for (uint32_t *p = b; p < e; ++p) u += p == b || dst[p[-1]] != dst[*p];
+        ucount[v] = u;
+    }
+    g.rp.assign(nn + 1, 0);
+    for (size_t v = 0; v < nn; ++v) g.rp[v + 1] = g.rp[v] + ucount[v];
+    g.col.resize(g.rp[nn]);
+    g.w.resize(g.rp[nn]);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (size_t v = 0; v < nn; ++v) {
+        const uint32_t *b = idx.data() + bstart[v], *e = idx.data() + bstart[v + 1];
+        uint64_t o = g.rp[v];
+        for (const uint32_t *p = b; p < e;) {
+            const int t = dst[*p];
+            double sum = w[*p];
+            const uint32_t *q = p + 1;
+            while (q < e && dst[*q] == t) sum += w[*q++];
+            g.col[o] = t;
+            g.w[o] = sum;
+            ++o;
+            p = q;
+        }
+    }
     for (size_t i = 0; i < m; ++i) {
         g.total_weight += w[i];
         g.in_wdeg[dst[i]] += w[i];
@@ -128,37 +163,123 @@ Graph from_arcs_n(int n, size_t m, const int *src, const int *dst, const double 
     return g;
 }
 
-Graph load_edge_list(std::istream &in)
+namespace {
+
+enum ChunkEnd { CHUNK_DONE, CHUNK_STOP, CHUNK_PARTIAL, CHUNK_BAD_WEIGHT, CHUNK_NEG };
+
+/* Triples of [p, end) in order; stops as the sequential loop does: a
+ * non-int id ends the input (CHUNK_STOP), whitespace only is CHUNK_DONE, ids
+ * with no weight before 'end' are CHUNK_PARTIAL. */
+ChunkEnd parse_chunk(const char *p, const char *end, std::vector<int> &src, std::vector<int> &dst,
+                     std::vector<double> &w)
 {
-    std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-    const char *p = text.data(), *end = text.data() + text.size();
-    std::vector<int> src, dst;
-    std::vector<double> w;
     for (;;) {
         long long s, d;
-        if (!parse_int(p, end, s)) break;
-        if (!parse_int(p, end, d)) break;
+        const char *start = p;
+        if (!parse_int(p, end, s)) {
+            while (start < end && is_space(*start)) ++start;
+            return start >= end ? CHUNK_DONE : CHUNK_STOP;
+        }
+        if (!parse_int(p, end, d)) {
+            const char *q = p;
+            while (q < end && is_space(*q)) ++q;
+            return q >= end ? CHUNK_PARTIAL : CHUNK_STOP;
+        }
         while (p < end && is_space(*p)) ++p;
-        if (p >= end) break;                      /* triple cut by the end of input */
-        char *q = nullptr;
-        errno = 0;
-        const double x = std::strtod(p, &q);
-        if (q == p || (q < end && !is_space(*q)) || !(x > 0))
-            throw std::invalid_argument("Edge weight must be greater than zero");
+        if (p >= end) return CHUNK_PARTIAL;
+        /* plain digit tokens convert exactly; anything else goes to strtod */
+        const char *q = p;
+        uint64_t iv = 0;
+        while (q < end && *q >= '0' && *q <= '9' && q - p < 15) iv = iv * 10 + (uint64_t)(*q++ - '0');
+        double x;
+        if (q > p && (q >= end || is_space(*q))) {
+            x = (double)iv;
+        } else {
+            char *r = nullptr;
+            errno = 0;
+            x = std::strtod(p, &r);
+            if (r == p || (r < end && !is_space(*r))) return CHUNK_BAD_WEIGHT;
+            q = r;
+        }
+        if (!(x > 0)) return CHUNK_BAD_WEIGHT;
+        if (s < 0 || d < 0) return CHUNK_NEG;
         p = q;
-        if (s < 0 || d < 0) throw std::invalid_argument("negative vertex id");
         src.push_back((int)s);
         dst.push_back((int)d);
         w.push_back(x);
     }
+}
+
+Graph graph_from_text(const std::string &text)
+{
+    const char *base = text.data(), *end = base + text.size();
+    const int nchunk = text.size() < (1u << 20) ? 1 : 64;
+    std::vector<const char *> cut(nchunk + 1);
+    cut[0] = base;
+    cut[nchunk] = end;
+    for (int k = 1; k < nchunk; ++k) {
+        const char *c = base + text.size() * k / nchunk;
+        while (c < end && *c != '\n') ++c;
+        cut[k] = c < end ? c + 1 : end;
+        if (cut[k] < cut[k - 1]) cut[k] = cut[k - 1];
+    }
+    std::vector<std::vector<int>> cs(nchunk), cd(nchunk);
+    std::vector<std::vector<double>> cw(nchunk);
+    std::vector<ChunkEnd> ce(nchunk);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < nchunk; ++k) ce[k] = parse_chunk(cut[k], cut[k + 1], cs[k], cd[k], cw[k]);
+    /* the chunks that count, in order, and where their triples go */
+    int used = 0;
+    for (int k = 0; k < nchunk; ++k) {
+        if (ce[k] == CHUNK_PARTIAL && k + 1 < nchunk) {
+            /* a triple across the cut: parse the whole text in one piece */
+            std::vector<int> src, dst;
+            std::vector<double> w;
+            const ChunkEnd e = parse_chunk(base, end, src, dst, w);
+            if (e == CHUNK_BAD_WEIGHT) throw std::invalid_argument("Edge weight must be greater than zero");
+            if (e == CHUNK_NEG) throw std::invalid_argument("negative vertex id");
+            return from_arcs(src.size(), src.data(), dst.data(), w.data());
+        }
+        used = k + 1;
+        if (ce[k] == CHUNK_BAD_WEIGHT) throw std::invalid_argument("Edge weight must be greater than zero");
+        if (ce[k] == CHUNK_NEG) throw std::invalid_argument("negative vertex id");
+        if (ce[k] == CHUNK_STOP || ce[k] == CHUNK_PARTIAL) break;   /* the input ends here */
+    }
+    std::vector<size_t> off((size_t)used + 1, 0);
+    for (int k = 0; k < used; ++k) off[k + 1] = off[k] + cs[k].size();
+    std::vector<int> src(off[used]), dst(off[used]);
+    std::vector<double> w(off[used]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < used; ++k) {
+        std::copy(cs[k].begin(), cs[k].end(), src.begin() + off[k]);
+        std::copy(cd[k].begin(), cd[k].end(), dst.begin() + off[k]);
+        std::copy(cw[k].begin(), cw[k].end(), w.begin() + off[k]);
+    }
     return from_arcs(src.size(), src.data(), dst.data(), w.data());
+}
+
+} // namespace
+
+Graph load_edge_list(std::istream &in)
+{
+    std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    return graph_from_text(text);
 }
 
 Graph load_edge_list_file(const std::string &path)
 {
-    std::ifstream in(path, std::ios::binary);
-    if (!in.is_open()) throw std::runtime_error("cannot open " + path);
-    return load_edge_list(in);
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string text;
+    char buf[1 << 16];
+    if (std::fseek(f, 0, SEEK_END) == 0) {
+        const long size = std::ftell(f);
+        if (size > 0) text.reserve((size_t)size);
+        std::rewind(f);
+    }
+    for (size_t r; (r = std::fread(buf, 1, sizeof buf, f)) > 0;) text.append(buf, r);
+    std::fclose(f);
+    return graph_from_text(text);
 }
 
 } // namespace fbchost
