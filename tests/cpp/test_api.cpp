@@ -169,13 +169,35 @@ static void gpu_checks(const char *dwg9, const char *cfg0)
     CHECK(err <= 1e-9 * scale);
 }
 
+/* Loads an edge list through the drop-in graph and writes its forward stars
+ * ("u v w" per arc, vertices and neighbours ascending) or the exception. */
+static void load_dump(const char *path, const char *out)
+{
+    FILE *f = std::fopen(out, "w");
+    if (!f) throw std::runtime_error("cannot write " + std::string(out));
+    try {
+        std::ifstream in(path);
+        fastbc::DirectedWeightedGraph<int, double> g(in);
+        std::fprintf(f, "n %zu\n", g.vertices().size());
+        for (int u : g.vertices())
+            for (const auto &kv : g.forwardStar(u)) std::fprintf(f, "%d %d %.17g\n", u, kv.first, kv.second);
+    } catch (const std::exception &ex) {
+        std::fprintf(f, "THROW %s\n", ex.what());
+    }
+    std::fclose(f);
+}
+
 int main(int argc, char **argv)
 {
     try {
+        if (argc == 4 && std::string(argv[1]) == "load") {
+            load_dump(argv[2], argv[3]);
+            return 0;
+        }
         if (argc == 5 && std::string(argv[1]) == "cpu") cpu_checks(argv[2], argv[3], argv[4]);
         else if (argc == 4 && std::string(argv[1]) == "gpu") gpu_checks(argv[2], argv[3]);
         else {
-            std::fprintf(stderr, "usage: test_api cpu <dwg9> <cfg0> <louvain0> | gpu <dwg9> <cfg0>\n");
+            std::fprintf(stderr, "usage: test_api cpu <dwg9> <cfg0> <louvain0> | gpu <dwg9> <cfg0> | load <edges> <out>\n");
             return 2;
         }
     } catch (const std::exception &ex) {
