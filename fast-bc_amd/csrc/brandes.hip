@@ -33,15 +33,24 @@
 namespace fbcdev {
 
 constexpr int HEAVY_DEG = 96;   /* out-degree handled by the whole workgroup */
-/* Between the in-degree sweep and v's dependency, sd[2v+1] holds v's count
- * of unprocessed DAG predecessors (high 32 bits) and its tight-arc mask (low
- * 32 bits): bit j says whether out-arc j of v lies on a shortest path, for
- * out-degrees up to MASK_ARCS; MASK_NONE marks wider rows, whose arcs are
- * tested against the distances.  A Kahn step thus updates a successor's
- * sigma and count in one 16-byte line segment. */
+/* Between the in-degree sweep and v's dependency, the low 32 bits of
+ * sd[2v+1] hold v's tight-arc mask: bit j says whether out-arc j of v lies on
+ * a shortest path, for out-degrees up to MASK_ARCS; MASK_NONE marks wider
+ * rows, whose arcs are tested against the distances.
+ * v's count of unprocessed DAG predecessors sits beside sigma(v) in one of
+ * two forms.  Packed: sd[2v] = count << 48 | sigma, so a Kahn step adds
+ * sigma(u) and takes one predecessor off with a single 64-bit atomic, and a
+ * count reaching zero leaves sd[2v] = sigma(v).  Wide: sd[2v] = sigma alone,
+ * the count in the high half of sd[2v+1] (two atomics per step).  A source
+ * runs packed; a sigma of 2^48 or more or an in-degree of 2^16 or more, which
+ * the packed form cannot hold, sends the source's sigma sweep round again in
+ * the wide form. */
 constexpr uint32_t MASK_ARCS = 31;
 constexpr uint32_t MASK_NONE = 0xFFFFFFFFu;
-constexpr uint64_t PRED_ONE = 1ull << 32;   /* one predecessor in sd[2v+1] */
+constexpr uint64_t PRED_ONE = 1ull << 32;      /* wide: one predecessor in sd[2v+1] */
+constexpr uint64_t PK_ONE = 1ull << 48;        /* packed: one predecessor in sd[2v] */
+constexpr uint64_t PK_SIGMA = PK_ONE - 1;      /* packed: the sigma bits */
+constexpr uint32_t PK_CMAX = 0xFFFFu;          /* packed: largest count */
 constexpr int HEAVY_CAP = 64;
 /* Per team: RB arcs of one vertex have their loads and atomics issued
  * together; the ordered fold keeps up to SUCC successors in registers.  The
@@ -102,10 +111,11 @@ __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~u
 
 /* Per-source state.  key[v] packs the tentative distance (high 32 bits) with
  * a tag (low 32 bits): the round in which v was last queued.  sd[2v] is
- * sigma(v); sd[2v+1] holds v's predecessor count and tight-arc mask from the
- * in-degree sweep until the dependency phase replaces it by delta(v) (a
- * double, or a float pair, bit-cast), so a DAG successor's (sigma, delta)
- * share one 16-byte line segment. */
+ * sigma(v) (with the predecessor count in the packed form); sd[2v+1] holds
+ * v's tight-arc mask (and the count in the wide form) from the in-degree
+ * sweep until the dependency phase replaces it by delta(v) (a double, or a
+ * float pair, bit-cast), so a DAG successor's (sigma, delta) share one
+ * 16-byte line segment. */
 struct Slot {
     uint64_t *key;
     uint32_t *fstamp, *qa, *qb, *fa, *fb;
@@ -226,7 +236,8 @@ struct Ctl {
     uint32_t qcnt[3];     /* near-queue lengths by round mod 3 */
     uint32_t fcnt[2];     /* far-list lengths by window parity */
     uint32_t minv[3];     /* smallest far distance by window mod 3 */
-    uint32_t tail;        /* level-list length */
+    uint32_t lcnt[3];     /* level-list entries pushed, by level mod 3 */
+    uint32_t wide;        /* the packed sigma form overflowed */
 };
 
 /* LDS of one local group (a workgroup, or one wavefront of a wavefront team):
@@ -490,7 +501,6 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
     for (uint32_t v = r0; v < V.n; v += R) {
         S.key[v] = v == src ? 1u : (uint64_t)DINF << 32;   /* src: distance 0, queued for round 1 */
         S.fstamp[v] = 0;
-        S.sd[2 * (uint64_t)v] = 0;
     }
     if (t.leader()) {
         ast32<SC>(&c.qcnt[0], 0u);
@@ -501,6 +511,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
         ast32<SC>(&c.minv[0], DINF);
         ast32<SC>(&c.minv[1], DINF);
         ast32<SC>(&c.minv[2], DINF);
+        ast32<SC>(&c.wide, 0u);
         if (T::LQ)
             t.lq[0] = src;
         else
@@ -598,12 +609,58 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
     }
 }
 
+/* Kahn arrivals of one block: sigma(u) = su joins each listed successor
+ * w[k] (ok[k]) and one of its predecessors is taken off, the atomics of the
+ * block in flight together; a successor whose last predecessor this was joins
+ * the next level, at lbase + its place in that level.  Packed form: a sigma
+ * carry into the count, or more arrivals than the level list holds (only
+ * after such a carry or a count beyond PK_CMAX), raises *wide. */
+template <bool PACKED, int SC, int RB>
+__device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t (&w)[RB], const bool (&ok)[RB],
+                                       uint64_t su, uint32_t lbase, uint32_t *lcnt, uint32_t *wide,
+                                       uint32_t *err)
+{
+    if constexpr (PACKED) {
+        uint64_t old[RB];
+#pragma unroll
+        for (int k = 0; k < RB; ++k) old[k] = ok[k] ? aadd64<SC>(&S.sd[2 * (uint64_t)w[k]], su - PK_ONE) : 0;
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            if (!ok[k]) continue;
+            if ((old[k] & PK_SIGMA) + su > PK_SIGMA) ast32<SC>(wide, 1u);
+            if ((old[k] >> 48) == 1u) {
+                const uint32_t i = lbase + aadd32<SC>(lcnt, 1u);
+                if (i < n)
+                    S.qa[i] = w[k];
+                else
+                    ast32<SC>(wide, 1u);
+            }
+        }
+    } else {
+        uint64_t old[RB], cnt[RB];
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            old[k] = ok[k] ? aadd64<SC>(&S.sd[2 * (uint64_t)w[k]], su) : 0;
+            cnt[k] = ok[k] ? asub64<SC>(&S.sd[2 * (uint64_t)w[k] + 1], PRED_ONE) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            if (!ok[k]) continue;
+            if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
+            if ((cnt[k] >> 32) == 1u) S.qa[lbase + aadd32<SC>(lcnt, 1u)] = w[k];
+        }
+    }
+    (void)n;
+    (void)wide;
+    (void)err;
+}
+
 /* Push sigma(u) along u's tight arcs, given by its mask (bit j: arc e0 + j),
- * RB at a time; a successor whose last predecessor this was joins the level
- * list. */
-template <class T>
+ * RB at a time. */
+template <bool PACKED, class T>
 __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint64_t su, uint32_t e0,
-                                                  uint32_t mask, uint32_t *tail, uint32_t *err)
+                                                  uint32_t mask, uint32_t lbase, uint32_t *lcnt,
+                                                  uint32_t *wide, uint32_t *err)
 {
     constexpr int SC = T::SC, RB = T::RB;
     uint32_t m = mask;
@@ -619,29 +676,17 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
         uint32_t w[RB];
 #pragma unroll
         for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
-        uint64_t old[RB], cnt[RB];
-#pragma unroll
-        for (int k = 0; k < RB; ++k) {
-            old[k] = ok[k] ? aadd64<SC>(&S.sd[2 * (uint64_t)w[k]], su) : 0;
-            cnt[k] = ok[k] ? asub64<SC>(&S.sd[2 * (uint64_t)w[k] + 1], PRED_ONE) : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < RB; ++k) {
-            if (!ok[k]) continue;
-            if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
-            if ((cnt[k] >> 32) == 1u) S.qa[aadd32<SC>(tail, 1u)] = w[k];
-        }
+        arrive<PACKED, SC, RB>(S, V.n, w, ok, su, lbase, lcnt, wide, err);
     }
 }
 
 /* Push sigma(u) along the DAG arcs among [e0, e1) (every step-th), testing
  * each against the distances, RB at a time with the loads and atomics of a
- * block in flight together; a successor whose last predecessor this was
- * joins the level list. */
-template <class T>
+ * block in flight together. */
+template <bool PACKED, class T>
 __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t du, uint64_t su,
-                                           uint32_t e0, uint32_t e1, uint32_t step, uint32_t *tail,
-                                           uint32_t *err)
+                                           uint32_t e0, uint32_t e1, uint32_t step, uint32_t lbase,
+                                           uint32_t *lcnt, uint32_t *wide, uint32_t *err)
 {
     constexpr int SC = T::SC, RB = T::RB;
     for (uint32_t eb = e0; eb < e1; eb += RB * step) {
@@ -649,33 +694,30 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
 #pragma unroll
         for (int j = 0; j < RB; ++j) a[j] = eb + j * step < e1 ? V.arcs[eb + j * step] : make_uint2(0u, 0u);
         bool tight[RB];
+        uint32_t w[RB];
 #pragma unroll
-        for (int j = 0; j < RB; ++j)
+        for (int j = 0; j < RB; ++j) {
             tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dist_of<SC>(S, a[j].x);
-        uint64_t old[RB], cnt[RB];
-#pragma unroll
-        for (int j = 0; j < RB; ++j) {
-            old[j] = tight[j] ? aadd64<SC>(&S.sd[2 * (uint64_t)a[j].x], su) : 0;
-            cnt[j] = tight[j] ? asub64<SC>(&S.sd[2 * (uint64_t)a[j].x + 1], PRED_ONE) : 0;
+            w[j] = a[j].x;
         }
-#pragma unroll
-        for (int j = 0; j < RB; ++j) {
-            if (!tight[j]) continue;
-            if (old[j] + su < old[j]) atomicOr(err, ERR_SIGMA);
-            if ((cnt[j] >> 32) == 1u) S.qa[aadd32<SC>(tail, 1u)] = a[j].x;
-        }
+        arrive<PACKED, SC, RB>(S, V.n, w, tight, su, lbase, lcnt, wide, err);
     }
 }
 
-/* Phase 2: DAG in-degrees and tight-arc masks, then the Kahn sweep with
- * exact sigma.  Returns the number of levels; the level list is S.qa, the
- * level starts S.fa.  The in-degree and the mask go into sd[2v+1].  The
- * sweep walks the vertices in order, so the distance tests it makes read
- * neighbouring lines; the Kahn sweep and the dependency phase, which visit
- * vertices scattered, then touch only the successors' sigma lines. */
-template <class T>
-__device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
-                                uint32_t sort_cap, uint32_t *err, Counters &ct)
+/* Phase 2 in one sigma form: DAG in-degrees and tight-arc masks, then the
+ * Kahn sweep with exact sigma.  Returns the number of levels; the level list
+ * is S.qa, the level starts S.fa.  The sweep walks the vertices in order, so
+ * the distance tests it makes read neighbouring lines, and it writes every
+ * vertex's (sigma, mask) pair whole; the Kahn sweep and the dependency phase,
+ * which visit vertices scattered, then touch only the successors' sigma
+ * lines.  Level l's arrivals are counted in lcnt[(l + 1) % 3], which every
+ * thread reads after the level's barrier and the leader clears two levels
+ * later, so the threads agree on every level's extent with one barrier per
+ * level.  In the packed form *wide reports an overflow (the result is then
+ * void). */
+template <bool PACKED, class T>
+__device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, const T &t, uint32_t src,
+                                               uint32_t sort_cap, bool count_te, uint32_t *err, Counters &ct)
 {
     constexpr int SC = T::SC;
     const uint32_t r0 = t.rank(), R = t.size();
@@ -742,26 +784,38 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
 #pragma unroll
         for (int k = 0; k < SU; ++k) {
             const uint32_t v = v0 + k * R;
-            if (v >= V.n || dv[k] == DINF) continue;
-            const uint32_t mk = e1[k] - e0[k] > MASK_ARCS ? MASK_NONE : mask[k];
-            S.sd[2 * (uint64_t)v + 1] = ((uint64_t)cin[k] << 32) | mk;
+            if (v >= V.n) break;
+            ulonglong2 pair = make_ulonglong2(0, 0);   /* unreached: sigma 0 */
+            if (dv[k] != DINF) {
+                const uint32_t mk = e1[k] - e0[k] > MASK_ARCS ? MASK_NONE : mask[k];
+                if (PACKED) {
+                    if (cin[k] > PK_CMAX) ast32<SC>(&c.wide, 1u);
+                    pair.x = v == src ? 1u : (uint64_t)cin[k] << 48;
+                    pair.y = mk;
+                } else {
+                    pair.x = v == src ? 1u : 0u;
+                    pair.y = ((uint64_t)cin[k] << 32) | mk;
+                }
+            }
+            reinterpret_cast<ulonglong2 *>(S.sd)[v] = pair;
         }
     }
     Local &L = *t.loc;
-    if (te) atomicAdd(&L.te, te);
+    if (te && count_te) atomicAdd(&L.te, te);
     if (t.leader()) {
-        S.sd[2 * (uint64_t)src] = 1;
         S.qa[0] = src;
         S.fa[0] = 0;
         S.fa[1] = 1;
-        ast32<SC>(&c.tail, 1u);
+        ast32<SC>(&c.lcnt[1], 0u);
     }
     if (t.lleader()) L.nheavy[0] = L.nheavy[1] = 0;
     t.sync();
     uint32_t lb = 0, le = 1, nlev = 1;
     for (uint32_t lvl = 0; lb < le; ++lvl) {
         uint32_t *nh_cnt = &L.nheavy[lvl & 1];
+        uint32_t *lc = &c.lcnt[(lvl + 1) % 3];
         if (t.lleader()) L.nheavy[(lvl + 1) & 1] = 0;
+        if (t.leader()) ast32<SC>(&c.lcnt[(lvl + 2) % 3], 0u);
         if constexpr (IsWG<T>::value)
             if (le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, t.sortbuf);
         for (uint32_t i = lb + r0; i < le; i += R) {
@@ -776,25 +830,44 @@ __device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, con
             }
             const uint32_t mask = (uint32_t)ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
             if (mask != MASK_NONE)
-                push_sigma_masked<T>(V, S, sigma_of<SC>(S, u), e0, mask, &c.tail, err);
+                push_sigma_masked<PACKED, T>(V, S, sigma_of<SC>(S, u), e0, mask, le, lc, &c.wide, err);
             else
-                push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, &c.tail, err);
+                push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, le, lc, &c.wide,
+                                      err);
         }
         t.lsync();
         const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
         for (uint32_t h = 0; h < nh; ++h) {
             const uint32_t u = L.heavy[h];
-            push_sigma<T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + t.lrank(),
-                          V.rp[u + 1], t.lsize(), &c.tail, err);
+            push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + t.lrank(), V.rp[u + 1],
+                                  t.lsize(), le, lc, &c.wide, err);
         }
         t.sync();
         lb = le;
-        le = ald32<SC>(&c.tail);
+        const uint32_t ne = le + ald32<SC>(lc);
+        le = ne < V.n ? ne : V.n;   /* more only after a packed overflow */
         if (le > lb) {
             ++nlev;
             if (t.leader()) S.fa[nlev] = le;
         }
         if (t.aborted()) return 0;
+    }
+    return nlev;
+}
+
+/* Phase 2: the packed sigma form, and the wide one when the packed form
+ * overflowed.  The overflow flag is read after the packed form's last
+ * barrier, when no thread sets it any more; the next source's phase 1 clears
+ * it. */
+template <class T>
+__device__ __forceinline__ uint32_t phase_sigma(const View &V, const Slot S, const T &t, uint32_t src,
+                                                uint32_t sort_cap, uint32_t *err, Counters &ct)
+{
+    uint32_t nlev = sigma_form<true>(V, S, t, src, sort_cap, true, err, ct);
+    if (ald32<T::SC>(&t.ctl->wide)) {
+        if (t.leader()) ++ct.c[STAT_SIGMA_WIDE];
+        t.sync();
+        nlev = sigma_form<false>(V, S, t, src, sort_cap, false, err, ct);
     }
     if (t.leader()) ct.c[STAT_LEVELS] += nlev;
     return nlev;

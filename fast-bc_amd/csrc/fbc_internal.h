@@ -86,7 +86,8 @@ enum Stat : int {
     STAT_T_DIST = 7,      /* time in the distance phase, summed over sources */
     STAT_T_SIGMA = 8,     /* time in the sigma phase */
     STAT_T_DEP = 9,       /* time in the dependency phase */
-    STAT_COUNT = 10
+    STAT_SIGMA_WIDE = 10, /* sources whose sigma sweep ran again in the wide form */
+    STAT_COUNT = 11
 };
 
 /* Views of at most this many vertices run one source per wavefront (their

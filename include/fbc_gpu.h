@@ -226,7 +226,8 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * rounds, windows, expanded vertices, relaxations, improvements, far
  * insertions, DAG levels, then the nanoseconds spent in the distance, sigma and
  * dependency phases (per workgroup, so the sum exceeds the kernel time by the
- * number of workgroups).  Fills at most 'count' entries, returns how many it
+ * number of workgroups), and the sources whose sigma counts outgrew the packed
+ * 48-bit form and were counted again in 64 bits.  Fills at most 'count' entries, returns how many it
  * wrote.  Instrumentation only; no counterpart in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
 

@@ -242,6 +242,29 @@ def test_unit_grid_sigma_closed_form(gpu_ctx):
     assert all(int(sig[0][v]) == comb(int(r[v] + c[v]), int(r[v])) for v in range(n))
 
 
+def test_unit_strip_sigma_beyond_48_bits(gpu_ctx):
+    """A 6 x 4000 unit-weight bidirected strip: 24,000 vertices (a workgroup
+    per source) and sigma(0 -> (r, c)) = C(r + c, r) up to C(4004, 5), about
+    8.6e15: beyond 48 bits, so the sigma sweep starts packed and runs again in
+    64 bits; below 2^53, so the oracle's FP64 dependencies are exact too."""
+    from math import comb
+    C = 4000
+    n, (s, d, w) = F.generate_grid(6, C, 1, 7, road=False)
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    dist, sig = g.sssp_sigma([0])
+    assert gpu_ctx.last_counters()["sigma_wide"] == 1
+    r, c = np.divmod(np.arange(n), C)
+    assert np.array_equal(dist[0], (r + c).astype(np.uint64))
+    assert all(int(sig[0][v]) == comb(int(r[v] + c[v]), int(r[v])) for v in range(n))
+    check_sssp(g, og, [0, 3 * C + C // 2, n - 1])
+    srcs = np.array([0, 5, 2 * C + 7, n - 1], np.uint32)
+    dep = g.dep_batch(srcs)
+    for i, src in enumerate(srcs):
+        ref = og.single_source_dep(int(src))
+        assert np.array_equal(dep[i], ref), f"source {src}"
+        assert np.array_equal(g.single_source_dep(int(src)), ref), f"plain, source {src}"
+
+
 def test_rmat_exact_bc(gpu_ctx):
     n, (s, d, w) = F.generate_rmat(10, 16, seed=11)
     g, og = both_graphs(gpu_ctx, n, s, d, w)
