@@ -828,12 +828,21 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
                     continue;
                 }
             }
-            const uint32_t mask = (uint32_t)ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
+            /* u's sigma and mask: final since u joined the list */
+            uint64_t su;
+            uint32_t mask;
+            if constexpr (SC == __HIP_MEMORY_SCOPE_WORKGROUP) {
+                const ulonglong2 pr = sd_pair(S, u);   /* one 16-byte load */
+                su = pr.x;
+                mask = (uint32_t)pr.y;
+            } else {
+                su = sigma_of<SC>(S, u);
+                mask = (uint32_t)ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
+            }
             if (mask != MASK_NONE)
-                push_sigma_masked<PACKED, T>(V, S, sigma_of<SC>(S, u), e0, mask, le, lc, &c.wide, err);
+                push_sigma_masked<PACKED, T>(V, S, su, e0, mask, le, lc, &c.wide, err);
             else
-                push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), e0, e1, 1, le, lc, &c.wide,
-                                      err);
+                push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), su, e0, e1, 1, le, lc, &c.wide, err);
         }
         t.lsync();
         const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
