@@ -1328,7 +1328,7 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
            *dkv = nullptr;
     int32_t *dcw = nullptr;
     uint32_t *dcent = nullptr, *dnext = nullptr, *dcnt = nullptr, *dstart = nullptr, *dasg = nullptr,
-             *dmem = nullptr, *dact = nullptr;
+             *dmem = nullptr, *dact = nullptr, *duk = nullptr, *dnu = nullptr, *dfirst = nullptr;
     hipError_t e;
     int rc = 0;
     double ms_total = 0;
@@ -1343,6 +1343,9 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         (e = dalloc(&dcent, ks)) != hipSuccess || (e = dalloc(&dnext, ks)) != hipSuccess ||
         (e = dalloc(&dcnt, ks)) != hipSuccess || (e = dalloc(&dstart, ks)) != hipSuccess ||
         (e = dalloc(&dasg, ps)) != hipSuccess || (e = dalloc(&dmem, ps)) != hipSuccess ||
+        (e = dalloc(&duk, ks)) != hipSuccess || (e = dalloc(&dnu, nsets)) != hipSuccess ||
+        (e = dalloc(&dfirst, ps)) != hipSuccess ||
+        (e = hipMemsetAsync(dfirst, 0xFF, ps * sizeof(uint32_t), ctx->stream)) != hipSuccess ||
         (e = dalloc(&dG, gmax)) != hipSuccess)
         rc = hip_fail(ctx, e, "k-means setup");
     if (!rc) {
@@ -1370,6 +1373,9 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         p.msum = dmsum;
         p.cweights = dcw;
         p.active = dact;
+        p.uk = duk;
+        p.nu = dnu;
+        p.firstk = dfirst;
         (void)hipEventRecord(ctx->ev0, ctx->stream);
         for (const std::vector<uint32_t> &grp : groups) {
             /* Gram tiles, then the iteration tiles, in one device array */
@@ -1448,7 +1454,7 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         if (e == hipSuccess) e = hipMemcpy(cweights, dcw, ks * sizeof(int32_t), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "k-means run");
     }
-    for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact}) dfree(q);
+    for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact, duk, dnu, dfirst}) dfree(q);
     for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
     for (auto *q : {dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
     dfree(dcw);

@@ -30,6 +30,9 @@ struct KmParams {
     double *msum;                       /* per centroid and feature: sum over members */
     int32_t *cweights;                  /* per centroid, final */
     uint32_t *active;                   /* per set: 1 while iterating */
+    uint32_t *uk;                       /* per centroid slot: the set's distinct centroids, by first index */
+    uint32_t *nu;                       /* per set: how many distinct centroids */
+    uint32_t *firstk;                   /* per point: scratch of the distinct-centroid pass, all ~0 between uses */
     double min_variance;
 };
 

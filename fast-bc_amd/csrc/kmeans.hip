@@ -20,8 +20,11 @@
  *   k_km_gram    64 x 64 tiles of G (upper triangle, mirrored)
  *   k_km_init    farthest-point initialisation (PlusPlusKMeans.h:172-207),
  *                one 1024-thread workgroup per set
- *   k_km_assign  first-wins nearest centroid of every point, 256 points per
- *                workgroup
+ *   k_km_unique  the distinct centroids of every set, by first index (the
+ *                farthest-point start and members' moves repeat points, and a
+ *                repeat never wins the first-wins assignment)
+ *   k_km_assign  first-wins nearest centroid of every point over the distinct
+ *                centroids, 256 points per workgroup
  *   k_km_members member lists grouped by centroid, one 1024-thread
  *                workgroup per set
  *   k_km_msum    member sums per centroid (the means' numerators), coalesced
@@ -219,30 +222,65 @@ __global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
     }
 }
 
+/* The distinct centroids of each set (one workgroup per set): centroid k
+ * is kept when no smaller index holds the same point; uk lists the kept
+ * indices ascending, nu counts them.  A repeated centroid scores exactly as
+ * its first occurrence, so the first-wins assignment never picks it and the
+ * assignment may scan the distinct ones only. */
+__global__ __launch_bounds__(SWG) void k_km_unique(KmParams p)
+{
+    const uint32_t c = blockIdx.x;
+    if (!p.active[c]) return;
+    const uint32_t K = p.K[c];
+    const uint32_t *cent = p.cent + p.koff[c];
+    uint32_t *first = p.firstk + p.poff[c];
+    uint32_t *uk = p.uk + p.koff[c];
+    __shared__ uint32_t s_wave[NWAVE];
+    for (uint32_t k = threadIdx.x; k < K; k += SWG)
+        __hip_atomic_fetch_min(&first[cent[k]], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t per = (K + SWG - 1) / SWG, k0 = threadIdx.x * per;
+    uint32_t part = 0;
+    for (uint32_t k = k0; k < k0 + per && k < K; ++k)
+        part += __hip_atomic_load(&first[cent[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == k;
+    uint32_t total;
+    uint32_t pos = block_scan(part, s_wave, &total);
+    for (uint32_t k = k0; k < k0 + per && k < K; ++k)
+        if (__hip_atomic_load(&first[cent[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == k) uk[pos++] = k;
+    if (threadIdx.x == 0) p.nu[c] = total;
+    __syncthreads();
+    /* back to ~0 for the next pass */
+    for (uint32_t k = threadIdx.x; k < K; k += SWG)
+        __hip_atomic_store(&first[cent[k]], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* grid: one workgroup per 256-point tile; tile_cl / tile_p0 map tiles */
 __global__ __launch_bounds__(AWG) void k_km_assign(KmParams p, const uint32_t *tile_cl, const uint32_t *tile_p0)
 {
     const uint32_t c = tile_cl[blockIdx.x];
     if (!p.active[c]) return;
-    const uint32_t P = p.P[c], K = p.K[c];
+    const uint32_t P = p.P[c], NU = p.nu[c];
     const double *xn = p.xnorm + p.poff[c];
     const double *Gs = p.G + p.goff[c];
     const uint32_t *cent = p.cent + p.koff[c];
-    __shared__ uint32_t s_u[AWG];
+    const uint32_t *uk = p.uk + p.koff[c];
+    __shared__ uint32_t s_u[AWG], s_k[AWG];
     __shared__ double s_n[AWG];
     const uint32_t v = tile_p0[blockIdx.x] + threadIdx.x;
     const bool valid = v < P;
     double best = 0;
     uint32_t bk = 0xFFFFFFFFu;
-    for (uint32_t k0 = 0; k0 < K; k0 += AWG) {
+    for (uint32_t j0 = 0; j0 < NU; j0 += AWG) {
         __syncthreads();
-        if (k0 + threadIdx.x < K) {
-            const uint32_t u = cent[k0 + threadIdx.x];
+        if (j0 + threadIdx.x < NU) {
+            const uint32_t k = uk[j0 + threadIdx.x];
+            const uint32_t u = cent[k];
+            s_k[threadIdx.x] = k;
             s_u[threadIdx.x] = u;
             s_n[threadIdx.x] = xn[u];
         }
         __syncthreads();
-        const uint32_t kn = K - k0 < (uint32_t)AWG ? K - k0 : (uint32_t)AWG;
+        const uint32_t kn = NU - j0 < (uint32_t)AWG ? NU - j0 : (uint32_t)AWG;
         if (!valid) continue;
         /* score ||c||^2 - 2 c.x ranks the centroids as ||x - c||^2 does (exact);
          * centroids ascending: the first of equals is kept */
@@ -251,7 +289,7 @@ __global__ __launch_bounds__(AWG) void k_km_assign(KmParams p, const uint32_t *t
             const double sc = s_n[j] - 2.0 * Gs[(uint64_t)s_u[j] * P + v];
             if (bk == 0xFFFFFFFFu || sc < best) {
                 best = sc;
-                bk = k0 + j;
+                bk = s_k[j];
             }
         }
     }
@@ -277,6 +315,16 @@ __global__ __launch_bounds__(SWG) void k_km_members(KmParams p)
     __syncthreads();
     for (uint32_t v = threadIdx.x; v < P; v += SWG) atomicAdd(&s_cnt[asg[v]], 1u);
     __syncthreads();
+    /* member sums start from zero (rows of centroids without members are
+     * never read) */
+    {
+        const uint32_t D = p.D[c];
+        double *msum = p.msum + p.moff[c];
+        const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+        for (uint32_t k = wave; k < K; k += NWAVE)
+            if (s_cnt[k])
+                for (uint32_t f = lane; f < D; f += 64) msum[(uint64_t)k * D + f] = 0.0;
+    }
     {
         const uint32_t per = (K + SWG - 1) / SWG, k0 = threadIdx.x * per;
         uint32_t part = 0;
@@ -297,9 +345,6 @@ __global__ __launch_bounds__(SWG) void k_km_members(KmParams p)
         const uint32_t k = asg[v];
         members[s_start[k] + atomicAdd(&s_cnt[k], 1u)] = v;
     }
-    /* member sums start from zero */
-    double *msum = p.msum + p.moff[c];
-    for (uint64_t e = threadIdx.x; e < (uint64_t)K * p.D[c]; e += SWG) msum[e] = 0.0;
 }
 
 /* Member sums per centroid: a workgroup takes 64 member slots (grouped by
@@ -479,9 +524,11 @@ hipError_t km_launch_init(const KmParams &p, hipStream_t s)
 hipError_t km_launch_iteration(const KmParams &p, const KmTiles &t, uint32_t iteration,
                                uint32_t max_iteration, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_km_assign, dim3(t.npt), dim3(AWG), 0, s, p, t.pt_cl, t.pt_first);
+    hipLaunchKernelGGL(k_km_unique, dim3(p.ncl), dim3(SWG), 0, s, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_km_assign, dim3(t.npt), dim3(AWG), 0, s, p, t.pt_cl, t.pt_first);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_km_members, dim3(p.ncl), dim3(SWG), 0, s, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_km_msum, dim3(t.nsum), dim3(AWG), 0, s, p, t.sum);
