@@ -110,7 +110,8 @@ template <int SC> __device__ __forceinline__ uint64_t amax64(uint64_t *p, uint64
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
 /* Per-source state.  key[v] packs the tentative distance (high 32 bits) with
- * a tag (low 32 bits): the round in which v was last queued.  sd[2v] is
+ * a tag (low 32 bits): NEAR_TAG | r when v was last queued for near round r,
+ * or w when it was last put on the far list of window w.  sd[2v] is
  * sigma(v) (with the predecessor count in the packed form); sd[2v+1] holds
  * v's tight-arc mask (and the count in the wide form) from the in-degree
  * sweep until the dependency phase replaces it by delta(v) (a double, or a
@@ -118,9 +119,10 @@ __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~u
  * 16-byte line segment. */
 struct Slot {
     uint64_t *key;
-    uint32_t *fstamp, *qa, *qb, *fa, *fb;
+    uint32_t *qa, *qb, *fa, *fb;
     uint64_t *sd;
 };
+constexpr uint32_t NEAR_TAG = 0x80000000u;
 
 __host__ __device__ inline uint64_t slot_layout(uint32_t nmax, uint8_t *base, Slot *s)
 {
@@ -128,14 +130,13 @@ __host__ __device__ inline uint64_t slot_layout(uint32_t nmax, uint8_t *base, Sl
     const uint64_t w64 = align256((uint64_t)(nmax + 1) * 8);
     if (s) {
         s->key = (uint64_t *)(base);
-        s->fstamp = (uint32_t *)(base + w64);
-        s->qa = (uint32_t *)(base + w64 + w32);
-        s->qb = (uint32_t *)(base + w64 + 2 * w32);
-        s->fa = (uint32_t *)(base + w64 + 3 * w32);
-        s->fb = (uint32_t *)(base + w64 + 4 * w32);
-        s->sd = (uint64_t *)(base + w64 + 5 * w32);
+        s->qa = (uint32_t *)(base + w64);
+        s->qb = (uint32_t *)(base + w64 + w32);
+        s->fa = (uint32_t *)(base + w64 + 2 * w32);
+        s->fb = (uint32_t *)(base + w64 + 3 * w32);
+        s->sd = (uint64_t *)(base + w64 + 4 * w32);
     }
-    return 3 * w64 + 5 * w32;
+    return 3 * w64 + 4 * w32;
 }
 
 /* distance half of key[v] */
@@ -412,19 +413,18 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
         return;
     }
     const uint32_t nd = (uint32_t)nd64;
-    /* near candidates carry the round tag, far ones tag 0: key order is the
-     * distance order, and an equal distance never replaces an older tag */
-    const uint64_t nk = ((uint64_t)nd << 32) | (nd < thr ? nxt : 0u);
+    const uint64_t nk = ((uint64_t)nd << 32) | (nd < thr ? NEAR_TAG | nxt : win);
     if (nk >= ald64<SC>(&S.key[a.x])) return;
     const uint64_t old = amin64<SC>(&S.key[a.x], nk);
     if ((old >> 32) <= nd) return;
     ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
-        /* queued already in this round (with a larger distance): not again */
-        if ((uint32_t)old != nxt) push_near<T>(lnext, qnext, qn_cnt, a.x);
+        /* queued already for the next round (with a larger distance): not again */
+        if ((uint32_t)old != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, a.x);
     } else {
         amin32<SC>(far_min, nd);
-        if (amax32<SC>(&S.fstamp[a.x], win) < win) {
+        /* on this window's far list already: not again */
+        if ((uint32_t)old != win) {
             far[aadd32<SC>(far_cnt, 1u)] = a.x;
             ++ct.c[STAT_FARPUSH];
         }
@@ -458,7 +458,7 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
                     atomicOr(err, ERR_DIST);
                 } else {
                     const uint32_t nd = (uint32_t)nd64;
-                    nk[j] = ((uint64_t)nd << 32) | (nd < thr ? nxt : 0u);
+                    nk[j] = ((uint64_t)nd << 32) | (nd < thr ? NEAR_TAG | nxt : win);
                 }
             }
         }
@@ -473,10 +473,10 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
             if ((cur[j] >> 32) <= nd) continue;
             ++ct.c[STAT_IMPROVE];
             if (nd < thr) {
-                if ((uint32_t)cur[j] != nxt) push_near<T>(lnext, qnext, qn_cnt, to[j]);
+                if ((uint32_t)cur[j] != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, to[j]);
             } else {
                 amin32<SC>(far_min, nd);
-                if (amax32<SC>(&S.fstamp[to[j]], win) < win) {
+                if ((uint32_t)cur[j] != win) {
                     far[aadd32<SC>(far_cnt, 1u)] = to[j];
                     ++ct.c[STAT_FARPUSH];
                 }
@@ -499,8 +499,7 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
     const uint32_t r0 = t.rank(), R = t.size();
     Ctl &c = *t.ctl;
     for (uint32_t v = r0; v < V.n; v += R) {
-        S.key[v] = v == src ? 1u : (uint64_t)DINF << 32;   /* src: distance 0, queued for round 1 */
-        S.fstamp[v] = 0;
+        S.key[v] = v == src ? NEAR_TAG | 1u : (uint64_t)DINF << 32;   /* src: distance 0, queued for round 1 */
     }
     if (t.leader()) {
         ast32<SC>(&c.qcnt[0], 0u);
@@ -575,8 +574,8 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
          * this window set: a lower bound of the far distances (an entry may
          * since have become near), and at least thr, so the next window
          * [thr, m + delta) is never empty of progress.  Far entries are
-         * unique within a window (fstamp), so each is queued at most once and
-         * needs no round tag. */
+         * unique within a window (the key's window tag), so each is queued at
+         * most once and needs no round tag. */
         const uint32_t nfar = ald32<SC>(&c.fcnt[win & 1]);
         if (nfar == 0) break;
         const uint32_t m = ald32<SC>(&c.minv[win % 3]);
@@ -596,8 +595,10 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             if (d < nthr) {
                 push_near<T>(lqa, qa, &c.qcnt[rnd % 3], w);
             } else {
+                /* carried to the next window's list, tagged as on it */
                 amin32<SC>(&c.minv[nwin % 3], d);
-                if (amax32<SC>(&S.fstamp[w], nwin) < nwin) fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
+                ast64<SC>(&S.key[w], ((uint64_t)d << 32) | nwin);
+                fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
             }
         }
         t.sync();
@@ -1233,9 +1234,11 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
 }
 
 /* One wavefront per source: WAVES slots per workgroup, each wavefront pulls
- * its own tasks. */
+ * its own tasks.  Four wavefronts per SIMD at least (128 VGPRs): these
+ * launches are latency-bound, and the ordered cluster form would otherwise
+ * take 130 and run three. */
 template <int MODE, int PREC, bool ORDERED>
-__global__ __launch_bounds__(WG) void k_brandes_wave(BrandesParams p)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_brandes_wave(BrandesParams p)
 {
     __shared__ SharedWave sw;
     const uint32_t w = threadIdx.x / 64;

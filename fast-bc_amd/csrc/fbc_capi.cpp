@@ -159,8 +159,8 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
-    /* whole-graph slots (44 B per vertex each) are also bounded by memory:
-     * a third of the device (4 per CU of a 1M-vertex graph take 45 GB; a
+    /* whole-graph slots (40 B per vertex each) are also bounded by memory:
+     * a third of the device (4 per CU of a 1M-vertex graph take 41 GB; a
      * 4M-vertex graph gets ~2 per CU) */
     const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20 : ctx->total_mem / 3;
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
