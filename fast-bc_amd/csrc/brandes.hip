@@ -792,7 +792,7 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
                 if (PACKED) {
                     if (cin[k] > PK_CMAX) ast32<SC>(&c.wide, 1u);
                     pair.x = v == src ? 1u : (uint64_t)cin[k] << 48;
-                    pair.y = mk;
+                    pair.y = ((uint64_t)e0[k] << 32) | mk;   /* the row start rides along */
                 } else {
                     pair.x = v == src ? 1u : 0u;
                     pair.y = ((uint64_t)cin[k] << 32) | mk;
@@ -821,6 +821,22 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
             if (le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, t.sortbuf);
         for (uint32_t i = lb + r0; i < le; i += R) {
             const uint32_t u = S.qa[i];
+            /* u's sigma, mask (and, packed, row start): final since u joined
+             * the list */
+            uint64_t su, y;
+            if constexpr (SC == __HIP_MEMORY_SCOPE_WORKGROUP) {
+                const ulonglong2 pr = sd_pair(S, u);   /* one 16-byte load */
+                su = pr.x;
+                y = pr.y;
+            } else {
+                su = sigma_of<SC>(S, u);
+                y = ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
+            }
+            const uint32_t mask = (uint32_t)y;
+            if (PACKED && mask != MASK_NONE) {   /* at most MASK_ARCS arcs: never heavy */
+                push_sigma_masked<PACKED, T>(V, S, su, (uint32_t)(y >> 32), mask, le, lc, &c.wide, err);
+                continue;
+            }
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
             if (e1 - e0 > (uint32_t)HEAVY_DEG) {
                 const uint32_t h = atomicAdd(nh_cnt, 1u);
@@ -828,17 +844,6 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
                     L.heavy[h] = u;
                     continue;
                 }
-            }
-            /* u's sigma and mask: final since u joined the list */
-            uint64_t su;
-            uint32_t mask;
-            if constexpr (SC == __HIP_MEMORY_SCOPE_WORKGROUP) {
-                const ulonglong2 pr = sd_pair(S, u);   /* one 16-byte load */
-                su = pr.x;
-                mask = (uint32_t)pr.y;
-            } else {
-                su = sigma_of<SC>(S, u);
-                mask = (uint32_t)ald64<SC>(&S.sd[2 * (uint64_t)u + 1]);
             }
             if (mask != MASK_NONE)
                 push_sigma_masked<PACKED, T>(V, S, su, e0, mask, le, lc, &c.wide, err);
@@ -933,15 +938,16 @@ struct Acc<1> {
  * tight-arc mask the successors are known: d(w) = d(v) + w(v, w) and only
  * their (sigma, delta) pairs are read. */
 template <int PREC, bool ORDERED, int RB, int SUCC_REG>
-__device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v)
+__device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v, bool packed)
 {
     Acc<PREC> acc;
-    const uint32_t dv = dist_final(S, v);
     const ulonglong2 own = sd_pair(S, v);
     const uint64_t sv_int = own.x;
-    const uint32_t mask = (uint32_t)own.y;   /* its predecessor count is 0 here */
+    const uint32_t mask = (uint32_t)own.y;
     const double sv = (double)sv_int;
-    const uint32_t e0 = V.rp[v], e1 = V.rp[v + 1];
+    /* the packed sigma form left a masked row's start beside its mask (the
+     * wide form's high half is its spent predecessor count) */
+    const uint32_t e0 = packed && mask != MASK_NONE ? (uint32_t)(own.y >> 32) : V.rp[v];
     if (!ORDERED && mask != MASK_NONE) {
         /* ascending arc order, RB successors' loads in flight together */
         uint32_t m = mask;
@@ -966,6 +972,7 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         }
         return acc;
     }
+    const uint32_t dv = dist_final(S, v);   /* the masked unordered fold needs no distance */
     if (ORDERED && mask != MASK_NONE) {
         if (__popc(mask) <= SUCC_REG) {
             /* the successors' (sigma, delta) loads go out together; only the
@@ -1019,6 +1026,7 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         }
         return acc;
     }
+    const uint32_t e1 = V.rp[v + 1];
     if (!ORDERED) {
         /* ascending arc order, RB successors' loads in flight together */
         for (uint32_t eb = e0; eb < e1; eb += RB) {
@@ -1097,35 +1105,35 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
 {
     const uint32_t r0 = t.rank(), R = t.size();
     const double wgt = MODE == MODE_PIVOT ? (double)p.task_aux[task] : 1.0;
+    const bool packed = ald32<T::SC>(&t.ctl->wide) == 0;   /* which sigma form ran */
     for (int k = (int)nlev - 1; k >= 0; --k) {
         const uint32_t i0 = S.fa[k], i1 = S.fa[k + 1];
         for (uint32_t i = i0 + r0; i < i1; i += R) {
             const uint32_t v = S.qa[i];
             if (v == src) continue;
-            const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC>(V, S, v);
+            const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC>(V, S, v, packed);
             S.sd[2 * (uint64_t)v + 1] = acc.bits();
         }
         t.sync();
         if (t.aborted()) return;
     }
-    /* the source and unreached vertices have no dependency (their delta
-     * slots hold older values); OU vertices per thread have their loads in
-     * flight together */
+    /* the source has no dependency (its delta slot holds its mask); the sigma
+     * sweep left an unreached vertex's slot 0, a zero dependency in either
+     * precision.  OU vertices per thread have their loads in flight
+     * together */
     constexpr int OU = 4;
     for (uint32_t v0 = r0; v0 < V.n; v0 += OU * R) {
-        uint32_t d[OU];
         uint64_t b[OU];
 #pragma unroll
         for (int k = 0; k < OU; ++k) {
             const uint32_t v = v0 + k * R;
-            d[k] = v < V.n ? dist_final(S, v) : DINF;
             b[k] = v < V.n ? S.sd[2 * (uint64_t)v + 1] : 0;
         }
 #pragma unroll
         for (int k = 0; k < OU; ++k) {
             const uint32_t v = v0 + k * R;
             if (v >= V.n) break;
-            const double dv = v != src && d[k] != DINF ? Acc<PREC>::value_of(b[k]) : 0.0;
+            const double dv = v != src ? Acc<PREC>::value_of(b[k]) : 0.0;
             if (MODE == MODE_DEP) {
                 p.out_dep[(uint64_t)task * V.n + v] = dv;
             } else if (MODE == MODE_EXACT) {
