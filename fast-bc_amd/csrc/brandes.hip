@@ -221,6 +221,7 @@ struct View {
     const uint32_t *rp;
     const uint2 *rarcs;
     const uint32_t *rrp;
+    const uint2 *ell;   /* short rows of the view, or nullptr */
     uint32_t n;
 };
 
@@ -431,9 +432,55 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
     }
 }
 
-/* Relax the arcs [e0, e1) of u (distance du), RB at a time: the arc loads,
- * the key loads and the atomicMins of a block are independent, so they are
- * issued back to back and their latencies overlap. */
+/* Relax NB arcs of u (distance du; arc j taken when ok[j]): the key loads and
+ * the atomicMins of the block are independent, so they are issued back to
+ * back and their latencies overlap. */
+template <class T, int NB>
+__device__ __forceinline__ void relax_arcs(const Slot S, uint32_t du, const uint2 (&a)[NB], const bool (&ok)[NB],
+                                           uint32_t thr, uint32_t nxt, uint32_t win, uint32_t *lnext,
+                                           uint32_t *qnext, uint32_t *qn_cnt, uint32_t *far, uint32_t *far_cnt,
+                                           uint32_t *far_min, uint32_t *err, Counters &ct)
+{
+    constexpr int SC = T::SC;
+    uint32_t to[NB];
+    uint64_t nk[NB], cur[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        to[j] = a[j].x;
+        nk[j] = ~uint64_t(0);
+        if (ok[j]) {
+            const uint64_t nd64 = (uint64_t)du + a[j].y;
+            if (nd64 >= DINF) {
+                atomicOr(err, ERR_DIST);
+            } else {
+                const uint32_t nd = (uint32_t)nd64;
+                nk[j] = ((uint64_t)nd << 32) | (nd < thr ? NEAR_TAG | nxt : win);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) cur[j] = nk[j] != ~uint64_t(0) ? ald64<SC>(&S.key[to[j]]) : 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) cur[j] = nk[j] < cur[j] ? amin64<SC>(&S.key[to[j]], nk[j]) : 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        /* cur[j] is the key before our atomicMin (0 when none was issued) */
+        const uint32_t nd = (uint32_t)(nk[j] >> 32);
+        if ((cur[j] >> 32) <= nd) continue;
+        ++ct.c[STAT_IMPROVE];
+        if (nd < thr) {
+            if ((uint32_t)cur[j] != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, to[j]);
+        } else {
+            amin32<SC>(far_min, nd);
+            if ((uint32_t)cur[j] != win) {
+                far[aadd32<SC>(far_cnt, 1u)] = to[j];
+                ++ct.c[STAT_FARPUSH];
+            }
+        }
+    }
+}
+
+/* Relax the arcs [e0, e1) of u (distance du), RB at a time. */
 template <class T>
 __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32_t du, uint32_t e0,
                                              uint32_t e1, uint32_t thr, uint32_t nxt, uint32_t win,
@@ -441,47 +488,17 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
                                              uint32_t *far_cnt, uint32_t *far_min, uint32_t *err,
                                              Counters &ct)
 {
-    constexpr int SC = T::SC, RB = T::RB;
+    constexpr int RB = T::RB;
     ct.c[STAT_RELAX] += e1 - e0;
     for (uint32_t eb = e0; eb < e1; eb += RB) {
-        uint32_t to[RB];
-        uint64_t nk[RB], cur[RB];
+        uint2 a[RB];
+        bool ok[RB];
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
-            to[j] = 0;
-            nk[j] = ~uint64_t(0);
-            if (eb + j < e1) {
-                const uint2 a = V.arcs[eb + j];
-                const uint64_t nd64 = (uint64_t)du + a.y;
-                to[j] = a.x;
-                if (nd64 >= DINF) {
-                    atomicOr(err, ERR_DIST);
-                } else {
-                    const uint32_t nd = (uint32_t)nd64;
-                    nk[j] = ((uint64_t)nd << 32) | (nd < thr ? NEAR_TAG | nxt : win);
-                }
-            }
+            ok[j] = eb + j < e1;
+            a[j] = ok[j] ? V.arcs[eb + j] : make_uint2(0u, 0u);
         }
-#pragma unroll
-        for (int j = 0; j < RB; ++j) cur[j] = nk[j] != ~uint64_t(0) ? ald64<SC>(&S.key[to[j]]) : 0;
-#pragma unroll
-        for (int j = 0; j < RB; ++j) cur[j] = nk[j] < cur[j] ? amin64<SC>(&S.key[to[j]], nk[j]) : 0;
-#pragma unroll
-        for (int j = 0; j < RB; ++j) {
-            /* cur[j] is the key before our atomicMin (0 when none was issued) */
-            const uint32_t nd = (uint32_t)(nk[j] >> 32);
-            if ((cur[j] >> 32) <= nd) continue;
-            ++ct.c[STAT_IMPROVE];
-            if (nd < thr) {
-                if ((uint32_t)cur[j] != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, to[j]);
-            } else {
-                amin32<SC>(far_min, nd);
-                if ((uint32_t)cur[j] != win) {
-                    far[aadd32<SC>(far_cnt, 1u)] = to[j];
-                    ++ct.c[STAT_FARPUSH];
-                }
-            }
-        }
+        relax_arcs<T, RB>(S, du, a, ok, thr, nxt, win, lnext, qnext, qn_cnt, far, far_cnt, far_min, err, ct);
     }
 }
 
@@ -542,6 +559,24 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                 const uint32_t u = near_entry<T>(lqa, qa, i);
                 ++ct.c[STAT_VISITS];
                 const uint32_t du = dist_of<SC>(S, u);
+                if (V.ell) {
+                    /* a short row: its arcs in one 32-byte load pair, no row starts */
+                    const uint4 *row = reinterpret_cast<const uint4 *>(V.ell + (uint64_t)ELL_W * u);
+                    const uint4 h0 = row[0], h1 = row[1];
+                    if (h0.x != ELL_WIDE) {
+                        const uint2 a[ELL_W] = {make_uint2(h0.x, h0.y), make_uint2(h0.z, h0.w),
+                                                make_uint2(h1.x, h1.y), make_uint2(h1.z, h1.w)};
+                        bool ok[ELL_W];
+#pragma unroll
+                        for (int j = 0; j < (int)ELL_W; ++j) {
+                            ok[j] = a[j].x != ELL_NONE;
+                            ct.c[STAT_RELAX] += ok[j];
+                        }
+                        relax_arcs<T, ELL_W>(S, du, a, ok, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err,
+                                             ct);
+                        continue;
+                    }
+                }
                 const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
                 if (e1 - e0 > (uint32_t)HEAVY_DEG) {
                     const uint32_t h = atomicAdd(nh_cnt, 1u);
@@ -1171,10 +1206,12 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
         V.n = p.cl_n[c];
         V.rp = p.rp + ro;
         V.rrp = p.rrp + ro;
+        V.ell = p.ell ? p.ell + (uint64_t)ELL_W * p.cl_voff[c] : nullptr;
     } else {
         V.n = p.n;
         V.rp = p.rp;
         V.rrp = p.rrp;
+        V.ell = p.ell;
     }
     V.arcs = p.arcs;
     V.rarcs = p.rarcs;
@@ -1281,6 +1318,7 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     V.n = p.n;
     V.rp = p.rp;
     V.rrp = p.rrp;
+    V.ell = p.ell;
     V.arcs = p.arcs;
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];

@@ -59,7 +59,7 @@ struct fbc_graph {
     bool blocked = false;
     std::vector<uint32_t> perm;                 /* original id -> blocked id */
     uint32_t *d_perm = nullptr;
-    uint2 *d_barcs = nullptr, *d_brarcs = nullptr;
+    uint2 *d_barcs = nullptr, *d_brarcs = nullptr, *d_bell = nullptr;
     uint32_t *d_brp = nullptr, *d_brrp = nullptr;
 };
 
@@ -73,7 +73,7 @@ struct fbc_clusters {
     uint32_t *d_n = nullptr, *d_gid = nullptr, *d_bptr = nullptr, *d_blocal = nullptr,
              *d_vcl = nullptr, *d_rp = nullptr, *d_rrp = nullptr;
     uint64_t *d_rpoff = nullptr, *d_voff = nullptr, *d_vioff = nullptr, *d_doff = nullptr;
-    uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
+    uint2 *d_arcs = nullptr, *d_rarcs = nullptr, *d_ell = nullptr;
     /* VertexInfo rows left on the device by fbc_cluster_eval_resident */
     uint32_t *d_vil = nullptr;
     int32_t *d_vic = nullptr;
@@ -424,6 +424,7 @@ void fbc_graph_free(fbc_graph *g)
     dfree(g->d_rrp);
     dfree(g->d_perm);
     dfree(g->d_barcs);
+    dfree(g->d_bell);
     dfree(g->d_brarcs);
     dfree(g->d_brp);
     dfree(g->d_brrp);
@@ -436,6 +437,18 @@ void fbc_graph_free(fbc_graph *g)
  * 16-vertex blob's per-vertex state then shares cache lines, and blobs that
  * are close in the graph are close in memory at every scale (the way a
  * space-filling curve orders a grid). */
+/* Short row of one vertex (ELL_W entries at out): its arcs padded with
+ * ELL_NONE, or ELL_WIDE when it has more than ELL_W. */
+static void ell_row(uint2 *out, const uint2 *row, uint32_t deg)
+{
+    if (deg > fbcdev::ELL_W) {
+        out[0] = make_uint2(fbcdev::ELL_WIDE, 0u);
+        for (uint32_t j = 1; j < fbcdev::ELL_W; ++j) out[j] = make_uint2(fbcdev::ELL_NONE, 0u);
+        return;
+    }
+    for (uint32_t j = 0; j < fbcdev::ELL_W; ++j) out[j] = j < deg ? row[j] : make_uint2(fbcdev::ELL_NONE, 0u);
+}
+
 static void blocked_order(const fbc_graph *g, std::vector<uint32_t> &perm)
 {
     static const uint32_t SIZES[] = {4096, 256, 16};
@@ -535,8 +548,12 @@ static int ensure_blocked(fbc_ctx *ctx, fbc_graph *g)
         std::vector<uint2> barcs, brarcs;
         build(g->rp, g->col, g->w, brp, barcs);
         build(g->rrp, g->rsrc, g->rw, brrp, brarcs);
+        std::vector<uint2> bell((size_t)n * fbcdev::ELL_W);
+#pragma omp parallel for schedule(static, 4096)
+        for (uint32_t nu = 0; nu < n; ++nu) ell_row(&bell[(size_t)nu * fbcdev::ELL_W], &barcs[brp[nu]], brp[nu + 1] - brp[nu]);
         hipError_t e;
-        if ((e = upload(&g->d_barcs, barcs, ctx->stream)) != hipSuccess ||
+        if ((e = upload(&g->d_bell, bell, ctx->stream)) != hipSuccess ||
+            (e = upload(&g->d_barcs, barcs, ctx->stream)) != hipSuccess ||
             (e = upload(&g->d_brarcs, brarcs, ctx->stream)) != hipSuccess ||
             (e = upload(&g->d_brp, brp, ctx->stream)) != hipSuccess ||
             (e = upload(&g->d_brrp, brrp, ctx->stream)) != hipSuccess ||
@@ -556,6 +573,7 @@ static void use_blocked(BrandesParams &p, const fbc_graph *g)
     p.rp = g->d_brp;
     p.rarcs = g->d_brarcs;
     p.rrp = g->d_brrp;
+    p.ell = g->d_bell;
 }
 
 /* Output-row modes (sssp / dep): sources in chunks that fit a device budget. */
@@ -717,7 +735,7 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
         cl->gid.assign(cluster_verts, cluster_verts + total);
         cl->vcl.resize(total);
         std::vector<uint32_t> rp, rrp;
-        std::vector<uint2> arcs, rarcs;
+        std::vector<uint2> arcs, rarcs, ell((size_t)total * fbcdev::ELL_W);
         rp.reserve(total + ncl);
         rrp.reserve(total + ncl);
         uint64_t vi = 0;
@@ -744,6 +762,8 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
                         border = true;
                     }
                 }
+                ell_row(&ell[(size_t)(v0 + i) * fbcdev::ELL_W], arcs.data() + rp.back(),
+                        (uint32_t)(arcs.size() - rp.back()));
                 for (uint32_t e = g->rrp[v]; e < g->rrp[v + 1]; ++e) {
                     const uint32_t s = g->rsrc[e];
                     if (owner[s] == (int32_t)c) rarcs.push_back(make_uint2(lid[s], g->rw[e]));
@@ -768,6 +788,7 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
             (e = upload(&cl->d_rrp, rrp, ctx->stream)) != hipSuccess ||
             (e = upload(&cl->d_arcs, arcs, ctx->stream)) != hipSuccess ||
             (e = upload(&cl->d_rarcs, rarcs, ctx->stream)) != hipSuccess ||
+            (e = upload(&cl->d_ell, ell, ctx->stream)) != hipSuccess ||
             (e = upload(&cl->d_rpoff, cl->rpoff, ctx->stream)) != hipSuccess ||
             (e = upload(&cl->d_voff, cl->voff, ctx->stream)) != hipSuccess ||
             (e = upload(&cl->d_vioff, cl->vioff, ctx->stream)) != hipSuccess ||
@@ -813,6 +834,7 @@ void fbc_clusters_free(fbc_clusters *cl)
     dfree(cl->d_rp);
     dfree(cl->d_rrp);
     dfree(cl->d_arcs);
+    dfree(cl->d_ell);
     dfree(cl->d_rarcs);
     dfree(cl->d_rpoff);
     dfree(cl->d_voff);
@@ -903,6 +925,7 @@ static int cluster_eval(fbc_ctx *ctx, fbc_clusters *cl, double *intra_bc, uint32
             p.rarcs = cl->d_rarcs;
             p.rp = cl->d_rp;
             p.rrp = cl->d_rrp;
+            p.ell = cl->d_ell;
             p.nmax = cl->nmax;
             p.cl_n = cl->d_n;
             p.cl_rpoff = cl->d_rpoff;

@@ -8,6 +8,13 @@
 namespace fbcdev {
 
 constexpr uint32_t DINF = 0xFFFFFFFFu;     /* unreached */
+/* Short rows, copied beside the CSR: ELL_W arcs per vertex, padded with
+ * ELL_NONE targets; a longer row is marked ELL_WIDE in its first entry and
+ * read from the CSR.  The distance phase reads a vertex's arcs in one
+ * 32-byte row instead of a row-start line and an arc line. */
+constexpr uint32_t ELL_W = 4;
+constexpr uint32_t ELL_NONE = 0xFFFFFFFFu;
+constexpr uint32_t ELL_WIDE = 0xFFFFFFFEu;
 constexpr int WG = 256;                    /* threads per source (4 wavefronts) */
 
 enum Mode : int {
@@ -31,6 +38,7 @@ struct BrandesParams {
     const uint32_t *rp;       /* row starts, n+1 */
     const uint2 *rarcs;       /* reverse arcs {src, w} */
     const uint32_t *rrp;
+    const uint2 *ell;         /* short rows (nullptr: the CSR only); cluster views at ELL_W * cl_voff */
     uint32_t n;
 
     /* cluster views: concatenated local CSRs (row starts are absolute) */
