@@ -734,48 +734,80 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
         cl->bptr.assign((size_t)ncl + 1, 0);
         cl->gid.assign(cluster_verts, cluster_verts + total);
         cl->vcl.resize(total);
-        std::vector<uint32_t> rp, rrp;
-        std::vector<uint2> arcs, rarcs, ell((size_t)total * fbcdev::ELL_W);
-        rp.reserve(total + ncl);
-        rrp.reserve(total + ncl);
+        /* per position (cluster-major, ascending ids): internal out- and
+         * in-degrees and the border flag, then offsets by prefix sums, then the
+         * rows filled in parallel; rp/rrp blocks are n_c + 1 entries per
+         * cluster, row starts absolute */
+        std::vector<uint32_t> odeg(total), ideg(total);
+        std::vector<uint8_t> isb(total);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (uint32_t c = 0; c < ncl; ++c)
+            for (uint32_t i = cluster_ptr[c]; i < cluster_ptr[c + 1]; ++i) {
+                const uint32_t v = cluster_verts[i];
+                uint32_t od = 0, id = 0;
+                bool border = false;
+                for (uint32_t e = g->rp[v]; e < g->rp[v + 1]; ++e) {
+                    if (owner[g->col[e]] == (int32_t)c) ++od;
+                    else border = true;
+                }
+                for (uint32_t e = g->rrp[v]; e < g->rrp[v + 1]; ++e) {
+                    if (owner[g->rsrc[e]] == (int32_t)c) ++id;
+                    else border = true;
+                }
+                odeg[i] = od;
+                ideg[i] = id;
+                isb[i] = border;
+                cl->vcl[i] = c;
+            }
+        std::vector<uint64_t> aoff((size_t)total + 1, 0), roff((size_t)total + 1, 0);
+        for (uint64_t i = 0; i < total; ++i) {
+            aoff[i + 1] = aoff[i] + odeg[i];
+            roff[i + 1] = roff[i] + ideg[i];
+        }
+        if (aoff[total] > 0xFFFFFFFFull || roff[total] > 0xFFFFFFFFull) throw std::invalid_argument("too many cluster arcs");
+        std::vector<uint32_t> rp((size_t)total + ncl), rrp((size_t)total + ncl);
+        std::vector<uint2> arcs(aoff[total]), rarcs(roff[total]), ell((size_t)total * fbcdev::ELL_W);
+        std::vector<uint32_t> nb(ncl);
         uint64_t vi = 0;
         for (uint32_t c = 0; c < ncl; ++c) {
             const uint32_t v0 = cluster_ptr[c], nc = cluster_ptr[c + 1] - v0;
+            uint32_t cnt = 0;
+            for (uint32_t i = v0; i < v0 + nc; ++i) cnt += isb[i];
+            nb[c] = cnt;
             cl->n[c] = nc;
             cl->nmax = std::max(cl->nmax, nc);
             cl->voff[c] = v0;
-            cl->rpoff[c] = rp.size();
+            cl->rpoff[c] = (uint64_t)v0 + c;
             cl->vioff[c] = vi;
-            uint64_t internal = 0;
+            cl->edges[c] = aoff[v0 + nc] - aoff[v0];
+            cl->bptr[c + 1] = cl->bptr[c] + cnt;
+            vi += (uint64_t)nc * cnt;
+        }
+        cl->blocal.resize(cl->bptr[ncl]);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (uint32_t c = 0; c < ncl; ++c) {
+            const uint32_t v0 = cluster_ptr[c], nc = cluster_ptr[c + 1] - v0;
+            const uint64_t r0 = cl->rpoff[c];
+            uint32_t b = cl->bptr[c];
             for (uint32_t i = 0; i < nc; ++i) {
-                const uint32_t v = cluster_verts[v0 + i];
-                cl->vcl[v0 + i] = c;
-                rp.push_back((uint32_t)arcs.size());
-                rrp.push_back((uint32_t)rarcs.size());
-                bool border = false;
+                const uint64_t pos = (uint64_t)v0 + i;
+                const uint32_t v = cluster_verts[pos];
+                rp[r0 + i] = (uint32_t)aoff[pos];
+                rrp[r0 + i] = (uint32_t)roff[pos];
+                uint64_t a = aoff[pos], r = roff[pos];
                 for (uint32_t e = g->rp[v]; e < g->rp[v + 1]; ++e) {
                     const uint32_t t = g->col[e];
-                    if (owner[t] == (int32_t)c) {
-                        arcs.push_back(make_uint2(lid[t], g->w[e]));
-                        ++internal;
-                    } else {
-                        border = true;
-                    }
+                    if (owner[t] == (int32_t)c) arcs[a++] = make_uint2(lid[t], g->w[e]);
                 }
-                ell_row(&ell[(size_t)(v0 + i) * fbcdev::ELL_W], arcs.data() + rp.back(),
-                        (uint32_t)(arcs.size() - rp.back()));
                 for (uint32_t e = g->rrp[v]; e < g->rrp[v + 1]; ++e) {
-                    const uint32_t s = g->rsrc[e];
-                    if (owner[s] == (int32_t)c) rarcs.push_back(make_uint2(lid[s], g->rw[e]));
-                    else border = true;
+                    const uint32_t s2 = g->rsrc[e];
+                    if (owner[s2] == (int32_t)c) rarcs[r++] = make_uint2(lid[s2], g->rw[e]);
                 }
-                if (border) cl->blocal.push_back(i);
+                ell_row(&ell[pos * fbcdev::ELL_W], arcs.data() + aoff[pos], odeg[pos]);
+                if (isb[pos]) cl->blocal[b++] = i;
             }
-            rp.push_back((uint32_t)arcs.size());
-            rrp.push_back((uint32_t)rarcs.size());
-            cl->edges[c] = internal;
-            cl->bptr[c + 1] = (uint32_t)cl->blocal.size();
-            vi += (uint64_t)nc * (cl->bptr[c + 1] - cl->bptr[c]);
+            rp[r0 + nc] = (uint32_t)aoff[(uint64_t)v0 + nc];
+            rrp[r0 + nc] = (uint32_t)roff[(uint64_t)v0 + nc];
         }
         cl->vi_total = vi;
         hipError_t e;

@@ -45,6 +45,7 @@ constexpr int NWAVE = SWG / 64;
 constexpr int AWG = 256;    /* points per assignment workgroup */
 constexpr int SNAP_M = 64;  /* members per LDS stage of the snap */
 constexpr int SNAP_F = 64;  /* features per LDS stage of the snap */
+constexpr int KM_INIT_REG = 16;   /* points per thread kept in registers by the initialisation */
 
 /* sum over features of (a - b)^2 in feature order */
 __device__ __forceinline__ double sq_rows(const double *a, const double *b, uint32_t D)
@@ -155,10 +156,13 @@ __global__ __launch_bounds__(GWG) void k_km_gram(KmParams p, double *G, const ui
             }
 }
 
-__global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
+/* Farthest-point initialisation of one set, one workgroup.  Each thread
+ * keeps the running means of its points (v = tid + j * SWG) in registers
+ * when the set fits (REG), else in cdist. */
+template <bool REG>
+__device__ __forceinline__ void km_init_set(const KmParams &p, uint32_t c)
 {
-    const uint32_t c = blockIdx.x;
-    if (!p.active[c]) return;
+    constexpr int NJ = REG ? KM_INIT_REG : 1;
     const uint32_t P = p.P[c], K = p.K[c];
     const double *xn = p.xnorm + p.poff[c];
     const double *Gs = p.G + p.goff[c];
@@ -167,7 +171,17 @@ __global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
     __shared__ double s_val[NWAVE];
     __shared__ uint32_t s_idx[NWAVE];
     __shared__ uint32_t s_last;
-    for (uint32_t v = threadIdx.x; v < P; v += SWG) cd[v] = 0.0;
+    double cdr[NJ], xnr[NJ];
+    if (REG) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const uint32_t v = threadIdx.x + j * SWG;
+            cdr[j] = 0.0;
+            xnr[j] = v < P ? xn[v] : 0.0;
+        }
+    } else {
+        for (uint32_t v = threadIdx.x; v < P; v += SWG) cd[v] = 0.0;
+    }
     if (threadIdx.x == 0) {
         cent[0] = 0;
         s_last = 0;
@@ -182,13 +196,28 @@ __global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
         const double xl = xn[last];
         double best = -1.0;
         uint32_t bidx = 0xFFFFFFFFu;
-        for (uint32_t v = threadIdx.x; v < P; v += SWG) {
-            const double d = (xl + xn[v]) - 2.0 * row[v];
-            const double nv = cd[v] * q + d * pr;
-            cd[v] = nv;
-            if (nv > best) {            /* v ascending within a thread: first wins */
-                best = nv;
-                bidx = v;
+        if (REG) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const uint32_t v = threadIdx.x + j * SWG;
+                if (v < P) {
+                    const double d = (xl + xnr[j]) - 2.0 * row[v];
+                    cdr[j] = cdr[j] * q + d * pr;
+                    if (cdr[j] > best) {   /* v ascending within a thread: first wins */
+                        best = cdr[j];
+                        bidx = v;
+                    }
+                }
+            }
+        } else {
+            for (uint32_t v = threadIdx.x; v < P; v += SWG) {
+                const double d = (xl + xn[v]) - 2.0 * row[v];
+                const double nv = cd[v] * q + d * pr;
+                cd[v] = nv;
+                if (nv > best) {
+                    best = nv;
+                    bidx = v;
+                }
             }
         }
         /* largest value, smallest index among equals: the reference's scan
@@ -220,6 +249,16 @@ __global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(SWG) void k_km_init(KmParams p)
+{
+    const uint32_t c = blockIdx.x;
+    if (!p.active[c]) return;
+    if (p.P[c] <= (uint32_t)SWG * KM_INIT_REG)
+        km_init_set<true>(p, c);
+    else
+        km_init_set<false>(p, c);
 }
 
 /* The distinct centroids of each set (one workgroup per set): centroid k
