@@ -243,13 +243,15 @@ def test_unit_grid_sigma_closed_form(gpu_ctx):
 
 
 def test_unit_strip_sigma_beyond_48_bits(gpu_ctx):
-    """A 6 x 4000 unit-weight bidirected strip: 24,000 vertices (a workgroup
-    per source) and sigma(0 -> (r, c)) = C(r + c, r) up to C(4004, 5), about
-    8.6e15: beyond 48 bits, so the sigma sweep starts packed and runs again in
-    64 bits; below 2^53, so the oracle's FP64 dependencies are exact too."""
+    """A 5 x 14000 unit-weight bidirected strip: 70,000 vertices (a workgroup
+    per source) and sigma(0 -> (r, c)) = C(r + c, r) up to C(14003, 4), about
+    1.6e15: beyond 48 bits, so the sigma sweep starts packed and runs again in
+    64 bits; below 2^53, so the oracle's FP64 dependencies are exact too.  The
+    pivot phase over the same sources (one cluster, no intra-cluster term)
+    equals the weighted sum of their dependency rows."""
     from math import comb
-    C = 4000
-    n, (s, d, w) = F.generate_grid(6, C, 1, 7, road=False)
+    C = 14000
+    n, (s, d, w) = F.generate_grid(5, C, 1, 7, road=False)
     g, og = both_graphs(gpu_ctx, n, s, d, w)
     dist, sig = g.sssp_sigma([0])
     assert gpu_ctx.last_counters()["sigma_wide"] == 1
@@ -263,6 +265,11 @@ def test_unit_strip_sigma_beyond_48_bits(gpu_ctx):
         ref = og.single_source_dep(int(src))
         assert np.array_equal(dep[i], ref), f"source {src}"
         assert np.array_equal(g.single_source_dep(int(src)), ref), f"plain, source {src}"
+    wts = np.array([1, 3, 2, 5], np.uint32)
+    cl = g.clusters(comm_of=np.zeros(n, np.int64))
+    bc = cl.pivot_phase(srcs, wts, np.zeros(len(srcs), np.uint32), np.zeros(n), np.zeros(n))
+    expect = (dep * wts[:, None].astype(float)).sum(0)
+    assert np.allclose(bc, expect, rtol=1e-12, atol=0), np.abs(bc - expect).max()
 
 
 def test_rmat_exact_bc(gpu_ctx):
