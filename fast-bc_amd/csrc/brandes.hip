@@ -744,12 +744,13 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
  * Kahn sweep with exact sigma.  Returns the number of levels; the level list
  * is S.qa, the level starts S.fa.  The sweep walks the vertices in order, so
  * the distance tests it makes read neighbouring lines, and it writes every
- * vertex's (sigma, mask) pair whole; the Kahn sweep and the dependency phase,
- * which visit vertices scattered, then touch only the successors' sigma
- * lines.  Level l's arrivals are counted in lcnt[(l + 1) % 3], which every
- * thread reads after the level's barrier and the leader clears two levels
- * later, so the threads agree on every level's extent with one barrier per
- * level.  In the packed form *wide reports an overflow (the result is then
+ * vertex's (sigma, mask) pair whole (packed: with the row start beside the
+ * mask); the Kahn sweep and the dependency phase, which visit vertices
+ * scattered, then touch only the successors' sigma lines.  Level l's arrivals
+ * are counted in lcnt[(l + 1) % 3], which every thread reads after the
+ * level's barrier and the leader clears two levels later, so the threads
+ * agree on every level's extent with one barrier per level.  In the packed
+ * form the team's Ctl::wide flag reports an overflow (the result is then
  * void). */
 template <bool PACKED, class T>
 __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, const T &t, uint32_t src,
