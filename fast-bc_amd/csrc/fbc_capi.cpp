@@ -1354,13 +1354,19 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         ks += k[s];
         ms += (uint64_t)k[s] * nfeat[s];
     }
-    /* groups of sets whose Gram matrices fit the budget together */
+    /* groups of sets whose Gram matrices fit the budget together, largest
+     * sets first: a group's initialisation lasts as long as its largest set's
+     * sequential choice of centroids, so similar sizes share a group */
     const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30), 1);
     std::vector<std::vector<uint32_t>> groups;
     uint64_t gmax = 0;
     {
+        std::vector<uint32_t> by_size(nsets);
+        for (uint32_t s = 0; s < nsets; ++s) by_size[s] = s;
+        std::stable_sort(by_size.begin(), by_size.end(),
+                         [&](uint32_t a, uint32_t b) { return npoints[a] > npoints[b]; });
         uint64_t used = 0;
-        for (uint32_t s = 0; s < nsets; ++s) {
+        for (uint32_t s : by_size) {
             const uint64_t g = (uint64_t)npoints[s] * npoints[s];
             if (groups.empty() || (used + g) * 8 > budget) {
                 groups.emplace_back();
