@@ -256,7 +256,7 @@ struct Local {
  * queues, this round's and the next), the rest in the slot's global queue.
  * A round then reads its vertices from LDS instead of one more dependent
  * global load. */
-constexpr uint32_t LQ_WG = 1024, LQ_WAVE = 256;
+constexpr uint32_t LQ_WG = 1024, LQ_WAVE = 256, LQ_HALF = 128;
 
 struct Shared {
     Ctl ctl;
@@ -271,6 +271,13 @@ struct SharedWave {
     Ctl ctl[WAVES];
     Local loc[WAVES];
     uint32_t lq[WAVES][2 * LQ_WAVE];
+};
+
+/* Per-workgroup LDS of the half-wavefront kernel: HALVES teams. */
+struct SharedHalf {
+    Ctl ctl[HALVES];
+    Local loc[HALVES];
+    uint32_t lq[HALVES][2 * LQ_HALF];
 };
 
 /* Sort q[0, n) ascending through LDS (bitonic; n <= SORT_CAP_MAX), so that a
@@ -351,6 +358,29 @@ struct TeamWave {
     __device__ uint32_t lrank() const { return threadIdx.x & 63; }
     __device__ uint32_t lsize() const { return 64; }
     __device__ bool lleader() const { return (threadIdx.x & 63) == 0; }
+    __device__ void lsync() const { wave_sync(); }
+};
+
+/* One half wavefront (32 lanes) per source: the two sources of a wavefront
+ * run in lockstep, each on its own slot, counters and LDS; a branch on which
+ * they disagree runs each half in turn.  For small views whose rounds and
+ * levels hold a few dozen vertices, two sources then share each memory
+ * latency of the step. */
+struct TeamHalf {
+    static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
+    static constexpr uint32_t LQ = LQ_HALF;
+    Ctl *ctl;
+    Local *loc;
+    uint32_t *lq;
+    __device__ uint32_t rank() const { return threadIdx.x & 31; }
+    __device__ uint32_t size() const { return 32; }
+    __device__ bool leader() const { return (threadIdx.x & 31) == 0; }
+    __device__ void sync() const { wave_sync(); }
+    __device__ bool aborted() const { return false; }
+    __device__ uint32_t lrank() const { return threadIdx.x & 31; }
+    __device__ uint32_t lsize() const { return 32; }
+    __device__ bool lleader() const { return (threadIdx.x & 31) == 0; }
     __device__ void lsync() const { wave_sync(); }
 };
 
@@ -1304,6 +1334,29 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     flush_counters(p, t, ct);
 }
 
+/* One half wavefront per source: HALVES slots per workgroup; each half pulls
+ * its own tasks and broadcasts them from its first lane. */
+template <int MODE, int PREC, bool ORDERED>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_brandes_half(BrandesParams p)
+{
+    __shared__ SharedHalf sh;
+    const uint32_t h = threadIdx.x / 32;
+    if (blockIdx.x * HALVES + h >= p.nslots) return;   /* no slot behind this half */
+    Slot S;
+    slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * HALVES + h) * p.slot_bytes, &S);
+    const TeamHalf t{&sh.ctl[h], &sh.loc[h], sh.lq[h]};
+    if (t.lleader()) sh.loc[h].te = 0;
+    Counters ct;
+    for (;;) {
+        uint32_t task = 0;
+        if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
+        task = __shfl(task, (int)(threadIdx.x & 32), 64);
+        if (task >= p.ntasks) break;
+        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
+    }
+    flush_counters(p, t, ct);
+}
+
 /* ---- the plain kernel: one source per cooperative launch over the grid ---- */
 
 template <int MODE>
@@ -1340,7 +1393,11 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
 template <int MODE, int PREC, bool ORDERED>
 static hipError_t launch_one(const BrandesParams &p, uint32_t slots, hipStream_t s)
 {
-    if (wave_team(p.nmax))
+    const int kind = team_kind(p.nmax, p.half_max);
+    if (kind == TEAM_HALF)
+        hipLaunchKernelGGL((k_brandes_half<MODE, PREC, ORDERED>), dim3((slots + HALVES - 1) / HALVES), dim3(WG), 0, s,
+                           p);
+    else if (kind == TEAM_WAVE)
         hipLaunchKernelGGL((k_brandes_wave<MODE, PREC, ORDERED>), dim3((slots + WAVES - 1) / WAVES), dim3(WG), 0, s, p);
     else
         hipLaunchKernelGGL((k_brandes<MODE, PREC, ORDERED>), dim3(slots), dim3(WG), 0, s, p);

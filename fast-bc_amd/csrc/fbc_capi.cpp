@@ -27,6 +27,7 @@ struct fbc_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
+    uint32_t half_max = fbcdev::HALF_TEAM_MAX_N;   /* views up to this many vertices: half-wavefront teams */
     uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
     uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
     uint64_t class_key_mask = ~0ull;  /* class hashes kept (fewer bits: collisions, a test hook) */
@@ -155,7 +156,8 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     const uint64_t sb = fbcdev::slot_bytes(nmax);
     /* whole-graph slots: 4 per CU (throughput levels off near 3); a
      * wavefront-per-source workgroup holds 4 slots, 8 such workgroups per CU */
-    const uint32_t per_cu = fbcdev::wave_team(nmax) ? 32u : 4u;
+    const int kind = fbcdev::team_kind(nmax, ctx->half_max);
+    const uint32_t per_cu = kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 4u;
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
@@ -166,7 +168,8 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     /* the wavefront kernel's workgroups address WAVES slots each: hold whole
      * workgroups' worth */
-    if (fbcdev::wave_team(nmax)) slots = (slots + fbcdev::WAVES - 1) / fbcdev::WAVES * fbcdev::WAVES;
+    const uint32_t unit = kind == fbcdev::TEAM_HALF ? fbcdev::HALVES : kind == fbcdev::TEAM_WAVE ? fbcdev::WAVES : 1;
+    slots = (slots + unit - 1) / unit * unit;
     const size_t need = (size_t)slots * sb;
     if (need > ctx->scratch_bytes) {
         dfree(ctx->scratch);
@@ -206,6 +209,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
     p.scratch = ctx->scratch;
     p.slot_bytes = fbcdev::slot_bytes(p.nmax);
     p.nslots = slots;
+    p.half_max = ctx->half_max;
     p.delta = auto_delta(ctx, g);
     p.sort_cap = ctx->sort_cap;
     p.queue_head = ctx->d_queue;
@@ -341,6 +345,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
     else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
     else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
+    else if (!std::strcmp(name, "half_team_max")) ctx->half_max = std::min<uint32_t>(v, fbcdev::WAVE_TEAM_MAX_N);
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);

@@ -76,7 +76,8 @@ struct BrandesParams {
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
     uint32_t *grid_ctl;       /* plain kernel: traversal counters shared by the grid (16 words) */
     uint32_t sort_cap;        /* batched kernel: sort near queues / levels of up to this many vertices (0: off) */
-    uint32_t nslots;          /* slots the scratch holds (the wavefront kernel checks its slot against it) */
+    uint32_t nslots;          /* slots the scratch holds (the wavefront kernels check their slot against it) */
+    uint32_t half_max;        /* views up to this many vertices: half-wavefront teams */
 };
 
 /* Largest near queue or level the batched kernel sorts (LDS words). */
@@ -106,6 +107,18 @@ enum Stat : int {
 constexpr uint32_t WAVE_TEAM_MAX_N = 65536;
 constexpr int WAVES = WG / 64;             /* slots of a wavefront-per-source workgroup */
 inline bool wave_team(uint32_t nmax) { return nmax <= WAVE_TEAM_MAX_N; }
+/* Views of at most half_max vertices (an option; 0 = never) run one source
+ * per half wavefront: two sources share a wavefront's instruction stream.
+ * Measured: clusters of a few hundred vertices (65K-vertex grid) run their
+ * launch 16-30% faster that way; at ~5K vertices the wavefront form is 1%
+ * faster, and a whole 64K-vertex graph runs 2x slower. */
+constexpr uint32_t HALF_TEAM_MAX_N = 2048;
+constexpr int HALVES = WG / 32;            /* slots of a half-wavefront-per-source workgroup */
+enum TeamKind { TEAM_WG = 0, TEAM_WAVE = 1, TEAM_HALF = 2 };
+inline int team_kind(uint32_t nmax, uint32_t half_max)
+{
+    return nmax <= half_max ? TEAM_HALF : wave_team(nmax) ? TEAM_WAVE : TEAM_WG;
+}
 
 /* dst[v] += src[perm[v]] for v < n (sums of a relabelled run back to the
  * original numbering). */

@@ -137,6 +137,25 @@ def test_cfg0_cluster_eval_bitexact(cfg0):
     assert np.array_equal(bc, ref)
 
 
+@pytest.mark.parametrize("half_max", [0, 65536])
+def test_cluster_eval_team_forms_agree(gpu_ctx, half_max):
+    """The cluster launch in its wavefront form (half teams off) and with
+    every view on half wavefronts gives the same intra BC and VertexInfo
+    rows as the default (cfg0's clusters of a few hundred vertices run on
+    half wavefronts by default)."""
+    n, (s, d, w) = F.generate_config(0)
+    comm, _ = F.louvain(s, d, w, [1, 2, 3, 4], 0.01)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    cl = g.clusters(comm_of=comm)
+    bc, L, C = cl.evaluate()
+    gpu_ctx.set_option("half_team_max", half_max)
+    try:
+        bc2, L2, C2 = cl.evaluate()
+    finally:
+        gpu_ctx.set_option("half_team_max", 2048)
+    assert np.array_equal(bc, bc2) and np.array_equal(L, L2) and np.array_equal(C, C2)
+
+
 def test_cfg0_pivot_phase(cfg0):
     g, og = cfg0
     r = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=0.5, threads=1)
