@@ -161,22 +161,40 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
-    /* whole-graph slots (40 B per vertex each) are also bounded by memory:
-     * a third of the device (4 per CU of a 1M-vertex graph take 41 GB; a
-     * 4M-vertex graph gets ~2 per CU) */
-    const size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20 : ctx->total_mem / 3;
+    /* slots (40 B per vertex each) are also bounded by memory: half the
+     * device, less what is not free now (the current scratch aside) and a
+     * sixteenth kept for the caller (4 per CU of a 1M-vertex graph take 41 GB;
+     * a 4M-vertex graph gets ~3 per CU, whose pivot phase runs 12% faster than
+     * with a third of the device) */
+    size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20 : ctx->total_mem / 2;
+    if (!ctx->scratch_mb) {
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+            const size_t avail = freeb + ctx->scratch_bytes, reserve = ctx->total_mem / 16;
+            budget = std::min(budget, avail > 2 * reserve ? avail - reserve : avail / 2);
+        }
+    }
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
-    /* the wavefront kernel's workgroups address WAVES slots each: hold whole
-     * workgroups' worth */
+    /* the wavefront kernels' workgroups address WAVES (HALVES) slots each:
+     * hold whole workgroups' worth */
     const uint32_t unit = kind == fbcdev::TEAM_HALF ? fbcdev::HALVES : kind == fbcdev::TEAM_WAVE ? fbcdev::WAVES : 1;
     slots = (slots + unit - 1) / unit * unit;
-    const size_t need = (size_t)slots * sb;
-    if (need > ctx->scratch_bytes) {
+    if ((size_t)slots * sb > ctx->scratch_bytes) {
         dfree(ctx->scratch);
         ctx->scratch_bytes = 0;
-        hipError_t e = hipMalloc((void **)&ctx->scratch, need);
-        if (e != hipSuccess) return hip_fail(ctx, e, "scratch allocation");
-        ctx->scratch_bytes = need;
+        /* fewer slots when the device cannot hold them all */
+        for (;;) {
+            const size_t need = (size_t)slots * sb;
+            hipError_t e = hipMalloc((void **)&ctx->scratch, need);
+            if (e == hipSuccess) {
+                ctx->scratch_bytes = need;
+                break;
+            }
+            (void)hipGetLastError();
+            ctx->scratch = nullptr;
+            if (e != hipErrorOutOfMemory || slots <= unit) return hip_fail(ctx, e, "scratch allocation");
+            slots = std::max(unit, (slots / 2 + unit - 1) / unit * unit);
+        }
     }
     *slots_out = slots;
     return 0;
@@ -1362,7 +1380,16 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
     /* groups of sets whose Gram matrices fit the budget together, largest
      * sets first: a group's initialisation lasts as long as its largest set's
      * sequential choice of centroids, so similar sizes share a group */
-    const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30), 1);
+    uint64_t budget = std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30);
+    {
+        /* and what is free now, less a sixteenth of the device */
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+            const uint64_t reserve = ctx->total_mem / 16;
+            budget = std::min<uint64_t>(budget, freeb > 2 * reserve ? freeb - reserve : freeb / 2);
+        }
+        budget = std::max<uint64_t>(budget, 1);
+    }
     std::vector<std::vector<uint32_t>> groups;
     uint64_t gmax = 0;
     {
