@@ -76,7 +76,8 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
  * locality; 0: on the graph as given), "scratch_mb" (device memory for the
  * sources in flight, in MiB; automatic: up to 4 whole-graph sources per CU
- * within a third of the device), "class_key_bits" (bits of the row hashes fbc_cluster_classes
+ * within half the device, less what is not free and a sixteenth kept back),
+ * "class_key_bits" (bits of the row hashes fbc_cluster_classes
  * keeps; 0: all 64; a few bits force collisions, a test hook),
  * "half_team_max" (views of at most this many vertices run one source per
  * half wavefront, two sources sharing its instruction stream; default 2048,
