@@ -221,7 +221,7 @@ struct View {
     const uint32_t *rp;
     const uint2 *rarcs;
     const uint32_t *rrp;
-    const uint2 *ell;   /* short rows of the view, or nullptr */
+    const uint4 *ell;   /* short rows of the view, or nullptr */
     uint32_t n;
 };
 
@@ -590,16 +590,16 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                 ++ct.c[STAT_VISITS];
                 const uint32_t du = dist_of<SC>(S, u);
                 if (V.ell) {
-                    /* a short row: its arcs in one 32-byte load pair, no row starts */
-                    const uint4 *row = reinterpret_cast<const uint4 *>(V.ell + (uint64_t)ELL_W * u);
-                    const uint4 h0 = row[0], h1 = row[1];
-                    if (h0.x != ELL_WIDE) {
-                        const uint2 a[ELL_W] = {make_uint2(h0.x, h0.y), make_uint2(h0.z, h0.w),
-                                                make_uint2(h1.x, h1.y), make_uint2(h1.z, h1.w)};
+                    /* a short row: its arcs in one 16-byte load, no row starts */
+                    const uint4 r = V.ell[u];
+                    if (r.x != ELL_WIDE) {
+                        const uint32_t e[ELL_W] = {r.x, r.y, r.z, r.w};
+                        uint2 a[ELL_W];
                         bool ok[ELL_W];
 #pragma unroll
                         for (int j = 0; j < (int)ELL_W; ++j) {
-                            ok[j] = a[j].x != ELL_NONE;
+                            ok[j] = e[j] != ELL_NONE;
+                            a[j] = make_uint2(u + (uint32_t)(int32_t)(int16_t)(e[j] & 0xFFFFu), e[j] >> 16);
                             ct.c[STAT_RELAX] += ok[j];
                         }
                         relax_arcs<T, ELL_W>(S, du, a, ok, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err,
@@ -1237,7 +1237,7 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
         V.n = p.cl_n[c];
         V.rp = p.rp + ro;
         V.rrp = p.rrp + ro;
-        V.ell = p.ell ? p.ell + (uint64_t)ELL_W * p.cl_voff[c] : nullptr;
+        V.ell = p.ell ? p.ell + p.cl_voff[c] : nullptr;
     } else {
         V.n = p.n;
         V.rp = p.rp;

@@ -60,7 +60,8 @@ struct fbc_graph {
     bool blocked = false;
     std::vector<uint32_t> perm;                 /* original id -> blocked id */
     uint32_t *d_perm = nullptr;
-    uint2 *d_barcs = nullptr, *d_brarcs = nullptr, *d_bell = nullptr;
+    uint2 *d_barcs = nullptr, *d_brarcs = nullptr;
+    uint4 *d_bell = nullptr;
     uint32_t *d_brp = nullptr, *d_brrp = nullptr;
 };
 
@@ -74,7 +75,8 @@ struct fbc_clusters {
     uint32_t *d_n = nullptr, *d_gid = nullptr, *d_bptr = nullptr, *d_blocal = nullptr,
              *d_vcl = nullptr, *d_rp = nullptr, *d_rrp = nullptr;
     uint64_t *d_rpoff = nullptr, *d_voff = nullptr, *d_vioff = nullptr, *d_doff = nullptr;
-    uint2 *d_arcs = nullptr, *d_rarcs = nullptr, *d_ell = nullptr;
+    uint2 *d_arcs = nullptr, *d_rarcs = nullptr;
+    uint4 *d_ell = nullptr;
     /* VertexInfo rows left on the device by fbc_cluster_eval_resident */
     uint32_t *d_vil = nullptr;
     int32_t *d_vic = nullptr;
@@ -460,16 +462,24 @@ void fbc_graph_free(fbc_graph *g)
  * 16-vertex blob's per-vertex state then shares cache lines, and blobs that
  * are close in the graph are close in memory at every scale (the way a
  * space-filling curve orders a grid). */
-/* Short row of one vertex (ELL_W entries at out): its arcs padded with
- * ELL_NONE, or ELL_WIDE when it has more than ELL_W. */
-static void ell_row(uint2 *out, const uint2 *row, uint32_t deg)
+/* Short row of vertex v (arcs row[0, deg)): (weight << 16) | (uint16)(target
+ * - v) per arc, padded with ELL_NONE, or ELL_WIDE when the row does not fit. */
+static uint4 ell_row(uint32_t v, const uint2 *row, uint32_t deg)
 {
-    if (deg > fbcdev::ELL_W) {
-        out[0] = make_uint2(fbcdev::ELL_WIDE, 0u);
-        for (uint32_t j = 1; j < fbcdev::ELL_W; ++j) out[j] = make_uint2(fbcdev::ELL_NONE, 0u);
-        return;
+    uint32_t e[fbcdev::ELL_W];
+    bool fits = deg <= fbcdev::ELL_W;
+    for (uint32_t j = 0; j < fbcdev::ELL_W; ++j) {
+        e[j] = fbcdev::ELL_NONE;
+        if (!fits || j >= deg) continue;
+        const int64_t d = (int64_t)row[j].x - (int64_t)v;
+        if (d < -32768 || d > 32767 || row[j].y > fbcdev::ELL_WMAX) {
+            fits = false;
+            continue;
+        }
+        e[j] = (row[j].y << 16) | (uint32_t)(uint16_t)(int16_t)d;
     }
-    for (uint32_t j = 0; j < fbcdev::ELL_W; ++j) out[j] = j < deg ? row[j] : make_uint2(fbcdev::ELL_NONE, 0u);
+    if (!fits) return make_uint4(fbcdev::ELL_WIDE, fbcdev::ELL_NONE, fbcdev::ELL_NONE, fbcdev::ELL_NONE);
+    return make_uint4(e[0], e[1], e[2], e[3]);
 }
 
 static void blocked_order(const fbc_graph *g, std::vector<uint32_t> &perm)
@@ -571,9 +581,9 @@ static int ensure_blocked(fbc_ctx *ctx, fbc_graph *g)
         std::vector<uint2> barcs, brarcs;
         build(g->rp, g->col, g->w, brp, barcs);
         build(g->rrp, g->rsrc, g->rw, brrp, brarcs);
-        std::vector<uint2> bell((size_t)n * fbcdev::ELL_W);
+        std::vector<uint4> bell(n);
 #pragma omp parallel for schedule(static, 4096)
-        for (uint32_t nu = 0; nu < n; ++nu) ell_row(&bell[(size_t)nu * fbcdev::ELL_W], &barcs[brp[nu]], brp[nu + 1] - brp[nu]);
+        for (uint32_t nu = 0; nu < n; ++nu) bell[nu] = ell_row(nu, &barcs[brp[nu]], brp[nu + 1] - brp[nu]);
         hipError_t e;
         if ((e = upload(&g->d_bell, bell, ctx->stream)) != hipSuccess ||
             (e = upload(&g->d_barcs, barcs, ctx->stream)) != hipSuccess ||
@@ -789,7 +799,8 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
         }
         if (aoff[total] > 0xFFFFFFFFull || roff[total] > 0xFFFFFFFFull) throw std::invalid_argument("too many cluster arcs");
         std::vector<uint32_t> rp((size_t)total + ncl), rrp((size_t)total + ncl);
-        std::vector<uint2> arcs(aoff[total]), rarcs(roff[total]), ell((size_t)total * fbcdev::ELL_W);
+        std::vector<uint2> arcs(aoff[total]), rarcs(roff[total]);
+        std::vector<uint4> ell(total);
         std::vector<uint32_t> nb(ncl);
         uint64_t vi = 0;
         for (uint32_t c = 0; c < ncl; ++c) {
@@ -826,7 +837,7 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
                     const uint32_t s2 = g->rsrc[e];
                     if (owner[s2] == (int32_t)c) rarcs[r++] = make_uint2(lid[s2], g->rw[e]);
                 }
-                ell_row(&ell[pos * fbcdev::ELL_W], arcs.data() + aoff[pos], odeg[pos]);
+                ell[pos] = ell_row(i, arcs.data() + aoff[pos], odeg[pos]);
                 if (isb[pos]) cl->blocal[b++] = i;
             }
             rp[r0 + nc] = (uint32_t)aoff[(uint64_t)v0 + nc];

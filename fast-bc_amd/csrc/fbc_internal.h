@@ -8,13 +8,17 @@
 namespace fbcdev {
 
 constexpr uint32_t DINF = 0xFFFFFFFFu;     /* unreached */
-/* Short rows, copied beside the CSR: ELL_W arcs per vertex, padded with
- * ELL_NONE targets; a longer row is marked ELL_WIDE in its first entry and
- * read from the CSR.  The distance phase reads a vertex's arcs in one
- * 32-byte row instead of a row-start line and an arc line. */
+/* Short rows, copied beside the CSR: a vertex v's arcs in one 16-byte row of
+ * ELL_W words, each (weight << 16) | (uint16)(target - v), padded with
+ * ELL_NONE; a row that does not fit (more than ELL_W arcs, a target more than
+ * 32767 ids away, a weight of 0xFFFF or more) is marked ELL_WIDE in its
+ * first word and read from the CSR.  The distance phase reads a vertex's arcs
+ * in one row instead of a row-start line and an arc line, and the rows of a
+ * whole graph take a quarter of its CSR arcs, so more of them stay in L2. */
 constexpr uint32_t ELL_W = 4;
 constexpr uint32_t ELL_NONE = 0xFFFFFFFFu;
 constexpr uint32_t ELL_WIDE = 0xFFFFFFFEu;
+constexpr uint32_t ELL_WMAX = 0xFFFEu;   /* largest weight a short row holds */
 constexpr int WG = 256;                    /* threads per source (4 wavefronts) */
 
 enum Mode : int {
@@ -38,7 +42,7 @@ struct BrandesParams {
     const uint32_t *rp;       /* row starts, n+1 */
     const uint2 *rarcs;       /* reverse arcs {src, w} */
     const uint32_t *rrp;
-    const uint2 *ell;         /* short rows (nullptr: the CSR only); cluster views at ELL_W * cl_voff */
+    const uint4 *ell;         /* short rows (nullptr: the CSR only); cluster views at cl_voff */
     uint32_t n;
 
     /* cluster views: concatenated local CSRs (row starts are absolute) */
