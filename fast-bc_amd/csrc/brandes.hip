@@ -681,10 +681,10 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
  * the next level, at lbase + its place in that level.  Packed form: a sigma
  * carry into the count, or more arrivals than the level list holds (only
  * after such a carry or a count beyond PK_CMAX), raises *wide. */
-template <bool PACKED, int SC, int RB>
+template <bool PACKED, int SC, int RB, uint32_t LQ>
 __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t (&w)[RB], const bool (&ok)[RB],
-                                       uint64_t su, uint32_t lbase, uint32_t *lcnt, uint32_t *wide,
-                                       uint32_t *err)
+                                       uint64_t su, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
+                                       uint32_t *wide, uint32_t *err)
 {
     if constexpr (PACKED) {
         uint64_t old[RB];
@@ -695,11 +695,13 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
             if (!ok[k]) continue;
             if ((old[k] & PK_SIGMA) + su > PK_SIGMA) ast32<SC>(wide, 1u);
             if ((old[k] >> 48) == 1u) {
-                const uint32_t i = lbase + aadd32<SC>(lcnt, 1u);
-                if (i < n)
+                const uint32_t x = aadd32<SC>(lcnt, 1u), i = lbase + x;
+                if (i < n) {
                     S.qa[i] = w[k];
-                else
+                    if (LQ && x < LQ) lqn[x] = w[k];
+                } else {
                     ast32<SC>(wide, 1u);
+                }
             }
         }
     } else {
@@ -713,19 +715,24 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
         for (int k = 0; k < RB; ++k) {
             if (!ok[k]) continue;
             if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
-            if ((cnt[k] >> 32) == 1u) S.qa[lbase + aadd32<SC>(lcnt, 1u)] = w[k];
+            if ((cnt[k] >> 32) == 1u) {
+                const uint32_t x = aadd32<SC>(lcnt, 1u);
+                S.qa[lbase + x] = w[k];
+                if (LQ && x < LQ) lqn[x] = w[k];
+            }
         }
     }
     (void)n;
     (void)wide;
     (void)err;
+    (void)lqn;
 }
 
 /* Push sigma(u) along u's tight arcs, given by its mask (bit j: arc e0 + j),
  * RB at a time. */
 template <bool PACKED, class T>
 __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint64_t su, uint32_t e0,
-                                                  uint32_t mask, uint32_t lbase, uint32_t *lcnt,
+                                                  uint32_t mask, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
                                                   uint32_t *wide, uint32_t *err)
 {
     constexpr int SC = T::SC, RB = T::RB;
@@ -742,7 +749,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
         uint32_t w[RB];
 #pragma unroll
         for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
-        arrive<PACKED, SC, RB>(S, V.n, w, ok, su, lbase, lcnt, wide, err);
+        arrive<PACKED, SC, RB, T::LQ>(S, V.n, w, ok, su, lbase, lcnt, lqn, wide, err);
     }
 }
 
@@ -752,7 +759,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
 template <bool PACKED, class T>
 __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t du, uint64_t su,
                                            uint32_t e0, uint32_t e1, uint32_t step, uint32_t lbase,
-                                           uint32_t *lcnt, uint32_t *wide, uint32_t *err)
+                                           uint32_t *lcnt, uint32_t *lqn, uint32_t *wide, uint32_t *err)
 {
     constexpr int SC = T::SC, RB = T::RB;
     for (uint32_t eb = e0; eb < e1; eb += RB * step) {
@@ -766,7 +773,7 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
             tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dist_of<SC>(S, a[j].x);
             w[j] = a[j].x;
         }
-        arrive<PACKED, SC, RB>(S, V.n, w, tight, su, lbase, lcnt, wide, err);
+        arrive<PACKED, SC, RB, T::LQ>(S, V.n, w, tight, su, lbase, lcnt, lqn, wide, err);
     }
 }
 
@@ -871,22 +878,31 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
     if (te && count_te) atomicAdd(&L.te, te);
     if (t.leader()) {
         S.qa[0] = src;
+        if (T::LQ) t.lq[0] = src;
         S.fa[0] = 0;
         S.fa[1] = 1;
         ast32<SC>(&c.lcnt[1], 0u);
     }
     if (t.lleader()) L.nheavy[0] = L.nheavy[1] = 0;
     t.sync();
+    /* a level's first LQ entries are also kept in LDS (the team's queue
+     * heads, free in this phase), so the next level reads them without a
+     * dependent global load; the dependency phase reads S.qa */
+    uint32_t *lqc = t.lq, *lqn = T::LQ ? t.lq + T::LQ : nullptr;
     uint32_t lb = 0, le = 1, nlev = 1;
     for (uint32_t lvl = 0; lb < le; ++lvl) {
         uint32_t *nh_cnt = &L.nheavy[lvl & 1];
         uint32_t *lc = &c.lcnt[(lvl + 1) % 3];
         if (t.lleader()) L.nheavy[(lvl + 1) & 1] = 0;
         if (t.leader()) ast32<SC>(&c.lcnt[(lvl + 2) % 3], 0u);
+        bool sorted = false;   /* a sorted level is read back from S.qa */
         if constexpr (IsWG<T>::value)
-            if (le - lb > 64 && le - lb <= sort_cap) sort_vertices(S.qa + lb, le - lb, t.sortbuf);
+            if (le - lb > 64 && le - lb <= sort_cap) {
+                sort_vertices(S.qa + lb, le - lb, t.sortbuf);
+                sorted = true;
+            }
         for (uint32_t i = lb + r0; i < le; i += R) {
-            const uint32_t u = S.qa[i];
+            const uint32_t u = T::LQ && !sorted && i - lb < T::LQ ? lqc[i - lb] : S.qa[i];
             /* u's sigma, mask (and, packed, row start): final since u joined
              * the list */
             uint64_t su, y;
@@ -900,7 +916,7 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
             }
             const uint32_t mask = (uint32_t)y;
             if (PACKED && mask != MASK_NONE) {   /* at most MASK_ARCS arcs: never heavy */
-                push_sigma_masked<PACKED, T>(V, S, su, (uint32_t)(y >> 32), mask, le, lc, &c.wide, err);
+                push_sigma_masked<PACKED, T>(V, S, su, (uint32_t)(y >> 32), mask, le, lc, lqn, &c.wide, err);
                 continue;
             }
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
@@ -912,18 +928,23 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
                 }
             }
             if (mask != MASK_NONE)
-                push_sigma_masked<PACKED, T>(V, S, su, e0, mask, le, lc, &c.wide, err);
+                push_sigma_masked<PACKED, T>(V, S, su, e0, mask, le, lc, lqn, &c.wide, err);
             else
-                push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), su, e0, e1, 1, le, lc, &c.wide, err);
+                push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), su, e0, e1, 1, le, lc, lqn, &c.wide, err);
         }
         t.lsync();
         const uint32_t nh = *nh_cnt < (uint32_t)HEAVY_CAP ? *nh_cnt : (uint32_t)HEAVY_CAP;
         for (uint32_t h = 0; h < nh; ++h) {
             const uint32_t u = L.heavy[h];
             push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), sigma_of<SC>(S, u), V.rp[u] + t.lrank(), V.rp[u + 1],
-                                  t.lsize(), le, lc, &c.wide, err);
+                                  t.lsize(), le, lc, lqn, &c.wide, err);
         }
         t.sync();
+        if (T::LQ) {
+            uint32_t *tq = lqc;
+            lqc = lqn;
+            lqn = tq;
+        }
         lb = le;
         const uint32_t ne = le + ald32<SC>(lc);
         le = ne < V.n ? ne : V.n;   /* more only after a packed overflow */
