@@ -150,7 +150,8 @@ class Context:
         self.check(lib().fbc_set_tuning(self.h, bucket_delta, slots))
 
     def set_option(self, name, value):
-        """Named tuning option: "delta", "slots", "frontier_sort"."""
+        """Named tuning option (fbc_set_option): "delta", "slots", "frontier_sort",
+        "relabel", "scratch_mb", "half_team_max", "class_key_bits"."""
         self.check(lib().fbc_set_option(self.h, name.encode(), int(value)))
 
     def last_stats(self):
