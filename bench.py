@@ -237,6 +237,7 @@ def main():
         plain = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, plain=True)
         plain_wall = time.perf_counter() - t
         plain_ms, plain_te = ctx.last_stats()
+        pc = ctx.last_counters()
         ctx.set_tuning(0, 0)
         batched = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, prec=F.PREC_F64)
         scale = max(np.abs(batched).max(), 1e-300)
@@ -254,7 +255,13 @@ def main():
             "sample": f"{k} of the run's {npiv} pivots, one cooperative launch of {args.plain_blocks} "
                       f"workgroups each; pivot phase scaled by {npiv}/{k}",
             "max_diff_vs_batched_fp64": agree,
+            "cooperative_launches": pc["plain_sources"],
+            "batched_fallback_sources": pc["plain_fallback_sources"],
         }
+        if pc["plain_fallback_sources"]:
+            log(f"plain form: {pc['plain_fallback_sources']} pivots ran on the batched kernel "
+                f"(cooperative launch refused, HIP status {pc['plain_refusal']}); no plain figure")
+            result["plain_fp64_form"]["value"] = None
         result["speedup_batched_fp32c_vs_plain_fp64"] = round(plain_total / total_s, 2)
         result["pivot_phase_speedup"] = round(plain_pivot_s / ph["pivot_phase_s"], 1)
         del g, cl, ctx

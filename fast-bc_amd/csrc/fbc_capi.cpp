@@ -32,7 +32,10 @@ struct fbc_ctx {
     uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
     uint64_t class_key_mask = ~0ull;  /* class hashes kept (fewer bits: collisions, a test hook) */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
-    uint64_t plain_fallbacks = 0;     /* plain-form runs finished on the batched kernel */
+    /* plain form, last call: sources run by cooperative launches, sources
+     * finished on the batched kernel after a refused first launch, and the
+     * HIP status of that refusal (0: none) */
+    uint64_t plain_sources = 0, plain_fallback_sources = 0, plain_refusal = 0;
     int cu_count = 0;
     size_t total_mem = 0;
     uint8_t *scratch = nullptr;
@@ -213,6 +216,14 @@ void base_params(BrandesParams &p, const fbc_graph *g)
     p.nmax = g->n;
 }
 
+/* Statuses with which the runtime refuses a cooperative launch for want of
+ * device resources rather than for a bad launch. */
+bool refused_cooperative(hipError_t e)
+{
+    return e == hipErrorCooperativeLaunchTooLarge || e == hipErrorNotSupported ||
+           e == hipErrorLaunchOutOfResources || e == hipErrorOutOfMemory;
+}
+
 /* Launch one run over p.ntasks tasks and wait for it: the persistent batched
  * kernel, or (plain) one cooperative grid launch per task. */
 int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p,
@@ -223,6 +234,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
         ctx->last_te = 0;
         return 0;
     }
+    ctx->plain_sources = ctx->plain_fallback_sources = ctx->plain_refusal = 0;
     uint32_t slots = 0;
     int rc = ensure_scratch(ctx, p.nmax, plain ? 1 : p.ntasks, &slots);
     if (rc) return rc;
@@ -248,29 +260,33 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
         if (blocks > want) blocks = want;
         p.grid_ctl = ctx->d_grid_ctl;
         uint32_t t = 0;
-        for (; t < p.ntasks; ++t)
-            if (fbcdev::launch_plain(mode, p, t, blocks, ctx->stream) != hipSuccess) {
-                (void)hipGetLastError();
-                break;
+        for (; t < p.ntasks; ++t) {
+            const hipError_t e = fbcdev::launch_plain(mode, p, t, blocks, ctx->stream);
+            if (e == hipSuccess) continue;
+            (void)hipGetLastError();
+            /* Only a refusal of the very first cooperative launch, with a
+             * status that says the device cannot take one now (another
+             * process holds its cooperative-launch resources), is finished on
+             * the batched kernel; any other failure is returned. */
+            if (t != 0 || !refused_cooperative(e)) {
+                (void)hipStreamSynchronize(ctx->stream);
+                return hip_fail(ctx, e, "cooperative launch of the plain form");
             }
+            ctx->plain_refusal = (uint64_t)e;
+            break;
+        }
+        ctx->plain_sources = t;
         if (t < p.ntasks) {
-            /* The runtime refused a cooperative launch (another process on
-             * the device holds its cooperative-launch resources).  The
-             * remaining sources run on the batched kernel, which computes the
-             * same values (FP64, reference successor order). */
-            ++ctx->plain_fallbacks;
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   /* queued plain launches use the scratch */
+            /* The remaining sources run on the batched kernel, which computes
+             * the same values (FP64, reference successor order); the count is
+             * reported by fbc_last_counters. */
+            ctx->plain_fallback_sources = p.ntasks - t;
             BrandesParams q = p;
-            q.task_src = p.task_src + t;
-            q.task_aux = p.task_aux ? p.task_aux + t : nullptr;
-            q.ntasks = p.ntasks - t;
-            if (mode == fbcdev::MODE_DEP) q.out_dep = p.out_dep + (uint64_t)t * p.n;
             uint32_t qslots = 0;
             rc = ensure_scratch(ctx, q.nmax, q.ntasks, &qslots);
             if (rc) return rc;
             q.scratch = ctx->scratch;
             q.nslots = qslots;
-            HIP_TRY(ctx, hipMemsetAsync(ctx->d_queue, 0, sizeof(uint32_t), ctx->stream));
             HIP_TRY(ctx, fbcdev::launch_brandes(mode, 0, true, q, qslots, ctx->stream));
         }
     } else {
@@ -1618,8 +1634,12 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count)
 {
     if (!ctx || !counters || count < 0) return FBC_E_ARG;
-    for (int i = 0; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];
-    return count < fbcdev::STAT_COUNT ? count : fbcdev::STAT_COUNT;
+    const uint64_t host[] = {ctx->plain_sources, ctx->plain_fallback_sources, ctx->plain_refusal};
+    constexpr int NH = (int)(sizeof(host) / sizeof(host[0]));
+    int i = 0;
+    for (; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];
+    for (; i < count && i < fbcdev::STAT_COUNT + NH; ++i) counters[i] = host[i - fbcdev::STAT_COUNT];
+    return i;
 }
 
 } // extern "C"

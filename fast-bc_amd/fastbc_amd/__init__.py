@@ -34,7 +34,8 @@ HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered
                 "fbc_host_release")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
-                 "ns_dist", "ns_sigma", "ns_dep", "sigma_wide")
+                 "ns_dist", "ns_sigma", "ns_dep", "sigma_wide",
+                 "plain_sources", "plain_fallback_sources", "plain_refusal")
 
 ERRORS = {
     -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",

@@ -196,10 +196,11 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
  * pivot with FP64 sums in the reference's successor order.  Same result
  * contract as fbc_pivot_phase with FBC_PREC_F64; the baseline the batched
  * form is measured against.  fbc_set_tuning's 'slots' sets the workgroups of
- * the grid (0: one per CU).  If the runtime refuses cooperative launches
- * (another process holds the device's cooperative-launch resources), this
- * and fbc_single_source_dep finish on the batched kernel, with the same
- * values. */
+ * the grid (0: one per CU).  If the runtime refuses the first cooperative
+ * launch of a call for want of device resources (another process holds the
+ * device's cooperative-launch resources), this and fbc_single_source_dep
+ * finish on the batched kernel, with the same values, and fbc_last_counters
+ * reports it; any other launch failure is returned as FBC_E_HIP. */
 int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
                           const uint32_t *weight, const uint32_t *pivot_cluster, uint32_t npiv,
                           const double *intra_bc, double *bc);
@@ -230,8 +231,12 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * rounds, windows, expanded vertices, relaxations, improvements, far
  * insertions, DAG levels, then the nanoseconds spent in the distance, sigma and
  * dependency phases (per workgroup, so the sum exceeds the kernel time by the
- * number of workgroups), and the sources whose sigma counts outgrew the packed
- * 48-bit form and were counted again in 64 bits.  Fills at most 'count' entries, returns how many it
+ * number of workgroups), the sources whose sigma counts outgrew the packed
+ * 48-bit form and were counted again in 64 bits; then, for the plain form
+ * (fbc_single_source_dep with FBC_PREC_F64, fbc_pivot_phase_plain), the
+ * sources run by cooperative whole-GPU launches, the sources finished on the
+ * batched kernel after a refused first launch, and the HIP status of that
+ * refusal (0: none).  Fills at most 'count' entries, returns how many it
  * wrote.  Instrumentation only; no counterpart in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
 

@@ -147,6 +147,15 @@ static void gpu_checks(const char *dwg9, const char *cfg0)
 
     DijkstraSSBrandesBC<int, double> ssb;                       /* DijkstraSSBrandesBC.cpp:26-28 */
     CHECK(ssb.singleSourceBrandes(0, full).size() == full->vertices().size());
+    {
+        uint64_t c[16] = {};
+        const int k = fbc_last_counters(gpuDevice().ctx(), c, 16);
+        CHECK(k >= 14);
+        std::printf("plain form: %llu source(s) on cooperative launches, %llu finished on the batched kernel "
+                    "(refusal status %llu)\n",
+                    (unsigned long long)c[11], (unsigned long long)c[12], (unsigned long long)c[13]);
+        CHECK(c[11] + c[12] == 1);
+    }
 
     /* main.cpp's 2-clustered composition vs the batched path on cfg0 */
     auto g0 = load(cfg0);

@@ -40,6 +40,14 @@ def cfg0(gpu_ctx):
     return both_graphs(gpu_ctx, n, s, d, w)
 
 
+def assert_plain_ran(ctx, nsrc):
+    """The last call ran every source on k_plain (cooperative whole-GPU
+    launches), none on the batched kernel."""
+    c = ctx.last_counters()
+    assert c["plain_fallback_sources"] == 0, f"cooperative launch refused (HIP status {c['plain_refusal']})"
+    assert c["plain_sources"] == nsrc, c
+
+
 def check_sssp(g, og, sources):
     dist, sig = g.sssp_sigma(sources)
     for i, s in enumerate(sources):
@@ -172,12 +180,14 @@ def test_cfg0_pivot_phase(cfg0):
     assert (err <= 1e-9 * np.abs(ref) + 1e-13 * mag).all(), (err / np.maximum(mag, 1)).max()
 
 
-def test_cfg0_plain_single_source_bitexact(cfg0):
+def test_cfg0_plain_single_source_bitexact(cfg0, gpu_ctx):
     """The plain form (one whole-GPU cooperative launch per source, FP64 in
     the reference's successor order) equals the oracle bit for bit."""
     g, og = cfg0
     for s in (0, 17, 2049, g.n - 1):
-        assert np.array_equal(g.single_source_dep(s), og.single_source_dep(s)), f"source {s}"
+        dep = g.single_source_dep(s)
+        assert_plain_ran(gpu_ctx, 1)
+        assert np.array_equal(dep, og.single_source_dep(s)), f"source {s}"
 
 
 def test_rmat_plain_single_source_heavy_rows(gpu_ctx):
@@ -188,15 +198,18 @@ def test_rmat_plain_single_source_heavy_rows(gpu_ctx):
     deg = np.bincount(s, minlength=n)
     assert deg.max() > 96
     for src in (int(deg.argmax()), 0, n - 1):
-        assert np.array_equal(g.single_source_dep(src), og.single_source_dep(src)), f"source {src}"
+        dep = g.single_source_dep(src)
+        assert_plain_ran(gpu_ctx, 1)
+        assert np.array_equal(dep, og.single_source_dep(src)), f"source {src}"
 
 
-def test_cfg0_pivot_phase_plain_matches_batched(cfg0):
+def test_cfg0_pivot_phase_plain_matches_batched(cfg0, gpu_ctx):
     g, og = cfg0
     r = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=0.5, threads=1)
     cl = g.clusters(comm_of=r["comm"])
     args = (r["pivots"], r["weights"], r["pivot_cluster"], r["intra"], r["intra"])
     plain = cl.pivot_phase(*args, plain=True)
+    assert_plain_ran(gpu_ctx, len(r["pivots"]))
     batched = cl.pivot_phase(*args)
     ref = r["bc"]
     dep = g.dep_batch(r["pivots"])
@@ -284,6 +297,7 @@ def test_unit_strip_sigma_beyond_48_bits(gpu_ctx):
         ref = og.single_source_dep(int(src))
         assert np.array_equal(dep[i], ref), f"source {src}"
         assert np.array_equal(g.single_source_dep(int(src)), ref), f"plain, source {src}"
+        assert_plain_ran(gpu_ctx, 1)
     wts = np.array([1, 3, 2, 5], np.uint32)
     cl = g.clusters(comm_of=np.zeros(n, np.int64))
     bc = cl.pivot_phase(srcs, wts, np.zeros(len(srcs), np.uint32), np.zeros(n), np.zeros(n))
