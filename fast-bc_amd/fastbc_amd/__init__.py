@@ -30,8 +30,8 @@ EXPORTS = (
 )
 
 # Every symbol include/fbc_host.h declares.
-HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_exact_bc_arcs",
-                "fbc_host_release")
+HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_clustered_bc_partition",
+                "fbc_exact_bc_arcs", "fbc_host_release")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
                  "ns_dist", "ns_sigma", "ns_dep", "sigma_wide",
@@ -111,6 +111,9 @@ def lib():
         L.fbc_clustered_bc.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, _u64p, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p, i32p,
                                        _f64p, _u32p, _u32p, _u32p, _f64p]
+        L.fbc_clustered_bc_partition.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, i32p,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _u32p,
+                                                 _u32p, _u32p, _f64p]
         L.fbc_exact_bc_arcs.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, ctypes.c_int, _f64p]
         L.fbc_host_release.argtypes = []
         L.fbc_host_release.restype = None
@@ -438,11 +441,13 @@ FORM_BATCHED, FORM_PLAIN, FORM_SELECT_ONLY = 0, 1, 2
 
 
 def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, device=0,
-                 form=FORM_BATCHED):
+                 form=FORM_BATCHED, partition=None):
     """End-to-end ClusteredBrandeBC::computeBC on the GPU; returns a dict.
     form: FORM_BATCHED (all pivots in one launch), FORM_PLAIN (one whole-GPU
     launch per pivot, FP64) or FORM_SELECT_ONLY (stop after pivot selection;
-    bc is then the intra-cluster BC)."""
+    bc is then the intra-cluster BC).  partition: a community index per
+    vertex in place of Louvain (negative: in no community; seeds and
+    precision are then unused)."""
     s, d, ww = _arcs_i32(src, dst, w)
     n = int(s.max()) + 1
     bc = np.zeros(n)
@@ -452,10 +457,20 @@ def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, d
     pw = np.zeros(n, np.uint32)
     pc = np.zeros(n, np.uint32)
     ph = np.zeros(8)
-    k = _host_check(lib().fbc_clustered_bc(device, len(s), s, d, ww, np.ascontiguousarray(seeds, np.uint64),
-                                           len(seeds), precision, kfrac, prec, form, bc, comm, intra, piv, pw,
-                                           pc, ph),
-                    "fbc_clustered_bc")
+    if partition is not None:
+        part = np.ascontiguousarray(partition, np.int32)
+        if part.shape != (n,):
+            raise ValueError(f"partition needs one entry per vertex ({n})")
+        k = _host_check(lib().fbc_clustered_bc_partition(device, len(s), s, d, ww, part, kfrac, prec, form, bc,
+                                                         intra, piv, pw, pc, ph), "fbc_clustered_bc_partition")
+        # communities as the library numbers them: non-empty indices, ascending
+        u = np.unique(part[part >= 0])
+        comm = np.where(part >= 0, np.searchsorted(u, part), -1).astype(np.int32)
+    else:
+        k = _host_check(lib().fbc_clustered_bc(device, len(s), s, d, ww, np.ascontiguousarray(seeds, np.uint64),
+                                               len(seeds), precision, kfrac, prec, form, bc, comm, intra, piv, pw,
+                                               pc, ph),
+                        "fbc_clustered_bc")
     return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=pw[:k], pivot_cluster=pc[:k],
                 phases=dict(zip(PHASES, ph.tolist())))
 

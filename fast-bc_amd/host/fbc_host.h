@@ -108,6 +108,10 @@ struct PhaseTimes {
 };
 
 struct ClusteredOptions {
+    /* the caller's partition, one community index per vertex (the reference
+     * takes any IGraphPartition, ClusteredBrandesBC.h:80-82); empty: Louvain
+     * over 'seeds' and 'precision' */
+    std::vector<int> partition;
     std::vector<uint64_t> seeds;
     double precision = 0.01;
     double kfrac = 0;            /* <= 0: plain VertexInfoPivotSelector */

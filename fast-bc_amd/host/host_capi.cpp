@@ -90,6 +90,61 @@ std::shared_ptr<Graph> graph_for(uint64_t m, const int32_t *src, const int32_t *
 
 } // namespace
 
+namespace {
+
+/* fbc_clustered_bc and fbc_clustered_bc_partition: Louvain over the seeds
+ * when comm_in is null, the caller's partition otherwise. */
+int clustered_entry(int device, uint64_t m, const int32_t *src, const int32_t *dst, const double *w,
+                    const int32_t *comm_in, const uint64_t *seeds, int nseeds, double precision, double kfrac,
+                    int prec, int form, double *bc, int32_t *comm_of, double *intra_bc, uint32_t *pivots,
+                    uint32_t *weights, uint32_t *pivot_cluster, double *phase_seconds)
+{
+    if ((m && (!src || !dst || !w)) || !bc) return FBC_E_ARG;
+    if (!comm_in && (!seeds || nseeds < 1)) return FBC_E_ARG;
+    if (prec != FBC_PREC_F64 && prec != FBC_PREC_F32C) return FBC_E_ARG;
+    if (form != FBC_FORM_BATCHED && form != FBC_FORM_PLAIN && form != FBC_FORM_SELECT_ONLY) return FBC_E_ARG;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(resident().mu);
+        const std::shared_ptr<Graph> gp = graph_for(m, src, dst, w);
+        const Graph &g = *gp;
+        Device &dev = device_for(device);
+        ClusteredOptions opt;
+        if (comm_in)
+            opt.partition.assign(comm_in, comm_in + g.n);
+        else
+            opt.seeds.assign(seeds, seeds + nseeds);
+        opt.precision = precision;
+        opt.kfrac = kfrac;
+        opt.prec = prec;
+        opt.form = form;
+        if (std::getenv("FBC_PROGRESS"))   /* progress lines on stderr */
+            opt.progress = [](int, const std::string &m) { std::fprintf(stderr, "[fbc] %s\n", m.c_str()); };
+        ClusteredRun r = clustered_bc(dev, g, opt);
+        std::memcpy(bc, r.bc.data(), r.bc.size() * sizeof(double));
+        if (comm_of)
+            for (size_t i = 0; i < r.comm.size(); ++i) comm_of[i] = r.comm[i];
+        if (intra_bc) std::memcpy(intra_bc, r.intra.data(), r.intra.size() * sizeof(double));
+        for (size_t i = 0; i < r.pivots.size(); ++i) {
+            if (pivots) pivots[i] = r.pivots[i];
+            if (weights) weights[i] = r.weights[i];
+            if (pivot_cluster) pivot_cluster[i] = r.pivot_cluster[i];
+        }
+        if (phase_seconds) {
+            phase_seconds[0] = r.t.louvain;
+            phase_seconds[1] = r.t.cluster;
+            phase_seconds[2] = r.t.pivots;
+            phase_seconds[3] = r.t.pivot_phase;
+            phase_seconds[4] = r.t.total;
+            phase_seconds[5] = r.t.cluster_te;
+            phase_seconds[6] = r.t.pivot_te;
+            phase_seconds[7] = r.t.pivot_kernel_ms;
+        }
+        return (int)r.pivots.size();
+    });
+}
+
+} // namespace
+
 extern "C" {
 
 void fbc_host_release(void)
@@ -156,44 +211,19 @@ int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *
                      int form, double *bc, int32_t *comm_of, double *intra_bc, uint32_t *pivots,
                      uint32_t *weights, uint32_t *pivot_cluster, double *phase_seconds)
 {
-    if ((m && (!src || !dst || !w)) || !seeds || nseeds < 1 || !bc) return FBC_E_ARG;
-    if (prec != FBC_PREC_F64 && prec != FBC_PREC_F32C) return FBC_E_ARG;
-    if (form != FBC_FORM_BATCHED && form != FBC_FORM_PLAIN && form != FBC_FORM_SELECT_ONLY) return FBC_E_ARG;
-    return guarded([&] {
-        std::lock_guard<std::mutex> lk(resident().mu);
-        const std::shared_ptr<Graph> gp = graph_for(m, src, dst, w);
-        const Graph &g = *gp;
-        Device &dev = device_for(device);
-        ClusteredOptions opt;
-        opt.seeds.assign(seeds, seeds + nseeds);
-        opt.precision = precision;
-        opt.kfrac = kfrac;
-        opt.prec = prec;
-        opt.form = form;
-        if (std::getenv("FBC_PROGRESS"))   /* progress lines on stderr */
-            opt.progress = [](int, const std::string &m) { std::fprintf(stderr, "[fbc] %s\n", m.c_str()); };
-        ClusteredRun r = clustered_bc(dev, g, opt);
-        std::memcpy(bc, r.bc.data(), r.bc.size() * sizeof(double));
-        if (comm_of)
-            for (size_t i = 0; i < r.comm.size(); ++i) comm_of[i] = r.comm[i];
-        if (intra_bc) std::memcpy(intra_bc, r.intra.data(), r.intra.size() * sizeof(double));
-        for (size_t i = 0; i < r.pivots.size(); ++i) {
-            if (pivots) pivots[i] = r.pivots[i];
-            if (weights) weights[i] = r.weights[i];
-            if (pivot_cluster) pivot_cluster[i] = r.pivot_cluster[i];
-        }
-        if (phase_seconds) {
-            phase_seconds[0] = r.t.louvain;
-            phase_seconds[1] = r.t.cluster;
-            phase_seconds[2] = r.t.pivots;
-            phase_seconds[3] = r.t.pivot_phase;
-            phase_seconds[4] = r.t.total;
-            phase_seconds[5] = r.t.cluster_te;
-            phase_seconds[6] = r.t.pivot_te;
-            phase_seconds[7] = r.t.pivot_kernel_ms;
-        }
-        return (int)r.pivots.size();
-    });
+    if (!seeds || nseeds < 1) return FBC_E_ARG;
+    return clustered_entry(device, m, src, dst, w, nullptr, seeds, nseeds, precision, kfrac, prec, form, bc, comm_of,
+                           intra_bc, pivots, weights, pivot_cluster, phase_seconds);
+}
+
+int fbc_clustered_bc_partition(int device, uint64_t m, const int32_t *src, const int32_t *dst, const double *w,
+                               const int32_t *comm_in, double kfrac, int prec, int form, double *bc,
+                               double *intra_bc, uint32_t *pivots, uint32_t *weights, uint32_t *pivot_cluster,
+                               double *phase_seconds)
+{
+    if (!comm_in) return FBC_E_ARG;
+    return clustered_entry(device, m, src, dst, w, comm_in, nullptr, 0, 0.0, kfrac, prec, form, bc, nullptr,
+                           intra_bc, pivots, weights, pivot_cluster, phase_seconds);
 }
 
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst, const double *w,

@@ -86,9 +86,26 @@ public:
         _opt.prec = prec;
     }
 
+    /* The batched path with the caller's partition strategy in place of
+     * Louvain, as ClusteredBrandeBC takes one (ClusteredBrandesBC.h:37-44,
+     * 80-82). */
+    GpuClusteredBrandesBC(std::shared_ptr<IGraphPartition<V, W>> gp, double kFrac = 0.0, int prec = FBC_PREC_F64)
+        : _gp(gp)
+    {
+        _opt.kfrac = kFrac;
+        _opt.prec = prec;
+    }
+
     std::vector<W> computeBC(const std::shared_ptr<const IGraph<V, W>> graph) override
     {
         auto g = bridge::host_graph_ptr<V, W>(*graph);
+        if (_gp) {
+            const std::vector<std::vector<V>> comms =
+                _gp->partitionGraph(std::static_pointer_cast<const IDegreeGraph<V, W>>(graph));
+            _opt.partition.assign((size_t)g->n, -1);
+            for (size_t c = 0; c < comms.size(); ++c)
+                for (const V &v : comms[c]) _opt.partition.at((size_t)v) = (int)c;
+        }
         _last = fbchost::clustered_bc(gpuDevice(), *g, _opt);
         return std::vector<W>(_last.bc.begin(), _last.bc.end());
     }
@@ -97,6 +114,7 @@ public:
     const fbchost::ClusteredRun &lastRun() const { return _last; }
 
 private:
+    std::shared_ptr<IGraphPartition<V, W>> _gp;
     fbchost::ClusteredOptions _opt;
     fbchost::ClusteredRun _last;
 };

@@ -345,8 +345,23 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     if (g.n == 0) return run;
 
     double t0 = now();
-    if (opt.progress) opt.progress(0, "Computing clusters with Louvain algorithm...");
-    const std::vector<std::vector<int>> comms = louvain(g, opt.seeds, opt.precision);
+    std::vector<std::vector<int>> comms;
+    if (!opt.partition.empty()) {
+        /* community c = the vertices whose index is c, ascending; empty
+         * indices dropped; a negative index leaves the vertex in no community
+         * (a partition strategy need not cover every vertex) */
+        if (opt.partition.size() != (size_t)g.n) throw std::invalid_argument("partition size differs from the graph's");
+        int k = 0;
+        for (int c : opt.partition) k = std::max(k, c + 1);
+        std::vector<std::vector<int>> by((size_t)k);
+        for (int v = 0; v < g.n; ++v)
+            if (opt.partition[(size_t)v] >= 0) by[(size_t)opt.partition[(size_t)v]].push_back(v);
+        for (auto &c : by)
+            if (!c.empty()) comms.push_back(std::move(c));
+    } else {
+        if (opt.progress) opt.progress(0, "Computing clusters with Louvain algorithm...");
+        comms = louvain(g, opt.seeds, opt.precision);
+    }
     run.t.louvain = now() - t0;
     const uint32_t ncl = (uint32_t)comms.size();
     run.t.clusters = ncl;

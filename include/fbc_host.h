@@ -58,6 +58,19 @@ int fbc_clustered_bc(int device, uint64_t m, const int32_t *src, const int32_t *
                      double *intra_bc, uint32_t *pivots, uint32_t *weights,
                      uint32_t *pivot_cluster, double *phase_seconds);
 
+/* fbc_clustered_bc with the caller's partition in place of Louvain, as
+ * ClusteredBrandeBC runs with any IGraphPartition strategy
+ * (brandes/ClusteredBrandesBC.h:80-82 calling IGraphPartition::partitionGraph,
+ * IGraphPartition.h:19): comm_in[n] gives each vertex's community (negative:
+ * in none, as a strategy need not cover every vertex); community c holds the
+ * vertices with index c in ascending order, the communities follow
+ * ascending c (empty ones dropped), and pivot_cluster counts the non-empty
+ * ones.  Other arguments as fbc_clustered_bc. */
+int fbc_clustered_bc_partition(int device, uint64_t m, const int32_t *src, const int32_t *dst,
+                               const double *w, const int32_t *comm_in, double kfrac, int prec,
+                               int form, double *bc, double *intra_bc, uint32_t *pivots,
+                               uint32_t *weights, uint32_t *pivot_cluster, double *phase_seconds);
+
 /* ExactBrandesBC::computeBC (brandes/ExactBrandesBC.h:46-95) from arcs. */
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,
                       const double *w, int prec, double *bc);

@@ -781,13 +781,30 @@ double seconds_since(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+/* ClusteredBrandeBC::computeBC (ClusteredBrandesBC.h:63-169).  The partition
+ * comes from Louvain (LouvainGraphPartition, as main.cpp composes it), or,
+ * when 'given' is not null, from the caller: the reference takes any
+ * IGraphPartition strategy (ClusteredBrandesBC.h:80-82), whose communities
+ * are vertex lists; here community c holds the vertices v with given[v] == c
+ * in ascending order, communities in ascending c, empty ones left out; a
+ * negative index leaves the vertex in no community. */
 int clustered(const orc_graph &g, const std::vector<uint64_t> &seeds, double precision,
               double kfrac, int threads, long long max_pivots, std::vector<double> &bc,
-              ClusteredRun &run)
+              ClusteredRun &run, const int *given = nullptr)
 {
     auto t0 = std::chrono::steady_clock::now();
     bc.assign(g.n, 0.0);
-    run.comms = louvain(g, seeds, precision);
+    if (given) {
+        int k = 0;
+        for (int v = 0; v < g.n; ++v) k = std::max(k, given[v] + 1);
+        std::vector<std::vector<int>> comms((size_t)k);
+        for (int v = 0; v < g.n; ++v)
+            if (given[v] >= 0) comms[(size_t)given[v]].push_back(v);
+        for (auto &c : comms)
+            if (!c.empty()) run.comms.push_back(std::move(c));
+    } else {
+        run.comms = louvain(g, seeds, precision);
+    }
     run.t_louvain = seconds_since(t0);
     t0 = std::chrono::steady_clock::now();
     const size_t nc = run.comms.size();
@@ -1126,6 +1143,39 @@ int orc_clustered_bc(const orc_graph *g, int nseeds, const uint64_t *seeds, doub
         }
     if (te) *te = run.te;
     return np;
+}
+
+int orc_clustered_bc_partition(const orc_graph *g, const int *comm_in, double kfrac, int threads, double *bc,
+                               double *intra_bc, int *pivots, int *weights, int *pivot_cluster)
+{
+    if (!comm_in || threads < 1) return -1;
+    std::vector<double> out;
+    ClusteredRun run;
+    int np = clustered(*g, {}, 0.0, kfrac, threads, LLONG_MAX, out, run, comm_in);
+    std::copy(out.begin(), out.end(), bc);
+    if (intra_bc) std::copy(run.intra.begin(), run.intra.end(), intra_bc);
+    int k = 0;
+    for (size_t c = 0; c < run.piv.size(); ++c)
+        for (size_t p = 0; p < run.piv[c].size(); ++p, ++k) {
+            if (pivots) pivots[k] = run.piv[c][p];
+            if (weights) weights[k] = run.piv_w[c][p];
+            if (pivot_cluster) pivot_cluster[k] = (int)c;
+        }
+    return np;
+}
+
+int orc_kmeans_pivots(int P, int nb, const double *vi_len, const int *vi_cnt, const int *weights, double kfrac,
+                      int *pivots, int *out_weights)
+{
+    if (P < 1 || nb < 0) return -1;
+    std::vector<int> rows((size_t)P), w(weights, weights + P), pr, pw;
+    for (int p = 0; p < P; ++p) rows[(size_t)p] = p;
+    std::vector<double> L(vi_len, vi_len + (size_t)P * nb);
+    std::vector<int> C(vi_cnt, vi_cnt + (size_t)P * nb);
+    kmeans_pivots(rows, w, nb, L, C, kfrac, pr, pw);
+    std::copy(pr.begin(), pr.end(), pivots);
+    std::copy(pw.begin(), pw.end(), out_weights);
+    return (int)pr.size();
 }
 
 int orc_clustered_pivots(const orc_graph *g, int nseeds, const uint64_t *seeds, double precision,

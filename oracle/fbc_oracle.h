@@ -105,6 +105,24 @@ int orc_clustered_bc(const orc_graph *g, int nseeds, const uint64_t *seeds,
                      int *comm_of, double *intra_bc, int *pivots, int *weights,
                      int *pivot_cluster, double *te);
 
+/* ClusteredBrandeBC::computeBC with a caller's partition in place of Louvain
+ * (the reference takes any IGraphPartition, ClusteredBrandesBC.h:80-82):
+ * community c = the vertices with comm_in == c, ascending, communities in
+ * ascending c (empty ones dropped); comm_in < 0: in no community.  Outputs as
+ * orc_clustered_bc (pivot_cluster indexes the non-empty communities).
+ * Returns the pivot count or a negative error. */
+int orc_clustered_bc_partition(const orc_graph *g, const int *comm_in, double kfrac, int threads,
+                               double *bc, double *intra_bc, int *pivots, int *weights, int *pivot_cluster);
+
+/* KMeansPivotSelector::selectPivots after the exact selector
+ * (KMeansPivotSelector.h:91-129): P exact pivots (rows 0..P-1 of vi_len /
+ * vi_cnt, class sizes in weights), k = max((int)(P * kfrac), 1),
+ * PlusPlusKMeans, then repeated centroids dropped keeping the first
+ * occurrence and its weight.  Fills pivots (row indices) and out_weights
+ * (capacity P); returns their count. */
+int orc_kmeans_pivots(int P, int nb, const double *vi_len, const int *vi_cnt, const int *weights, double kfrac,
+                      int *pivots, int *out_weights);
+
 /* Phases 1-3 of the clustered run only (partition, intra-cluster BC, pivots);
  * all outputs have capacity n.  Returns the pivot count. */
 int orc_clustered_pivots(const orc_graph *g, int nseeds, const uint64_t *seeds, double precision,

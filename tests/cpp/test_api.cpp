@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <sstream>
 #include <stdexcept>
 
 using namespace fastbc;
@@ -176,6 +177,34 @@ static void gpu_checks(const char *dwg9, const char *cfg0)
     std::printf("composition vs batched on cfg0: max |diff| %.3e (max |bc| %.3e), %zu pivots\n", err, scale,
                 batched.lastRun().pivots.size());
     CHECK(err <= 1e-9 * scale);
+
+    /* the batched path with a partition strategy of the caller's: a unit
+     * directed cycle of N = k + 1 vertices cut into the path {0..k-1} and
+     * {k}: BC = (N-1)(N-2)/2 + (1-k) i(k-1-i) on the path, (N-1)(N-2)/2 at k */
+    struct FixedPartition : IGraphPartition<int, double> {
+        std::vector<std::vector<int>> c;
+        std::vector<std::vector<int>> partitionGraph(std::shared_ptr<const IDegreeGraph<int, double>>) override
+        {
+            return c;
+        }
+    };
+    const int k = 12, N = k + 1;
+    std::stringstream cyc;
+    for (int i = 0; i < N; ++i) cyc << i << " " << (i + 1) % N << " 1\n";
+    auto gc = std::make_shared<DirectedWeightedGraph<int, double>>(cyc);
+    auto part = std::make_shared<FixedPartition>();
+    part->c.resize(2);
+    for (int i = 0; i < k; ++i) part->c[0].push_back(i);
+    part->c[1].push_back(k);
+    GpuClusteredBrandesBC<int, double> given(part);
+    auto bcc = given.computeBC(gc);
+    bool ok = bcc.size() == (size_t)N;
+    for (int i = 0; ok && i < N; ++i) {
+        const double e = (N - 1) * (N - 2) / 2.0 + (i < k ? (1.0 - k) * i * (k - 1 - i) : 0.0);
+        ok = std::fabs(bcc[i] - e) <= 1e-12 * std::fabs(e) + 1e-12;
+    }
+    std::printf("batched path with a given partition on a %d-cycle: %s\n", N, ok ? "closed form" : "MISMATCH");
+    CHECK(ok);
 }
 
 /* Loads an edge list through the drop-in graph and writes its forward stars

@@ -56,6 +56,9 @@ def lib():
                                        ctypes.POINTER(ctypes.c_double)]
         L.orc_clustered_pivots.argtypes = [P, ctypes.c_int, u64p, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int, i32p, f64p, i32p, i32p, i32p]
+        L.orc_clustered_bc_partition.argtypes = [P, i32p, ctypes.c_double, ctypes.c_int, f64p, f64p, i32p, i32p,
+                                                 i32p]
+        L.orc_kmeans_pivots.argtypes = [ctypes.c_int, ctypes.c_int, f64p, i32p, i32p, ctypes.c_double, i32p, i32p]
         L.orc_clustered_bc_sampled.argtypes = [P, ctypes.c_int, u64p, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int, ctypes.c_int, f64p, f64p]
         _lib = L
@@ -170,6 +173,31 @@ class Graph:
         assert k >= 0
         return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=w[:k],
                     pivot_cluster=pc[:k], te=te.value)
+
+
+def clustered_bc_partition(g, comm, kfrac=0.0, threads=1):
+    """Clustered BC with a caller's partition (community per vertex)."""
+    n = g.n
+    c = np.ascontiguousarray(comm, np.int32)
+    assert len(c) == n
+    bc, intra = np.zeros(n), np.zeros(n)
+    piv, w, pc = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    k = lib().orc_clustered_bc_partition(g.h, c, kfrac, threads, bc, intra, piv, w, pc)
+    assert k >= 0
+    return dict(bc=bc, intra=intra, pivots=piv[:k], weights=w[:k], pivot_cluster=pc[:k])
+
+
+def kmeans_pivots(vi_len, vi_cnt, weights, kfrac):
+    """KMeansPivotSelector after the exact selector: k-means over the P rows,
+    then the first occurrence of each repeated centroid kept."""
+    L = np.ascontiguousarray(vi_len, np.float64)
+    C = np.ascontiguousarray(vi_cnt, np.int32)
+    P, nb = L.shape
+    w = np.ascontiguousarray(weights, np.int32)
+    piv, pw = np.zeros(P, np.int32), np.zeros(P, np.int32)
+    k = lib().orc_kmeans_pivots(P, nb, L.reshape(-1), C.reshape(-1), w, kfrac, piv, pw)
+    assert k >= 0
+    return piv[:k], pw[:k]
 
 
 def clustered_pivots(g, seeds, precision=0.01, kfrac=0.0, threads=1):

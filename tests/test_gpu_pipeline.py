@@ -96,3 +96,65 @@ def test_rmat_exact_f32c(gpu_ctx):
         bc = F.exact_bc_arcs(s, d, w, prec=prec)
         rel = np.abs(bc - ref) / np.maximum(np.abs(ref), 1e-12 * ref.max())
         assert rel.max() <= tol, (prec, rel.max())
+
+
+def random_strong_arcs(seed, n):
+    rng = np.random.default_rng(seed)
+    arcs = {(v, (v + 1) % n): int(rng.integers(1, 4)) for v in range(n)}
+    for _ in range(2 * n):
+        a, b = (int(x) for x in rng.integers(0, n, 2))
+        if a != b:
+            arcs[(a, b)] = int(rng.integers(1, 4))
+    a = np.array([(x, y, c) for (x, y), c in sorted(arcs.items())])
+    return a[:, 0], a[:, 1], a[:, 2].astype(np.float64)
+
+
+@pytest.mark.parametrize("form,prec", [(F.FORM_BATCHED, F.PREC_F64), (F.FORM_BATCHED, F.PREC_F32C),
+                                       (F.FORM_PLAIN, F.PREC_F64)])
+def test_composition_closed_forms(gpu_ctx, form, prec):
+    """The product's composition with caller-given partitions (the cases of
+    tests/test_composition.py, which pin the oracle by the same closed
+    forms): a path community cut from a directed cycle,
+    (N-1)(N-2)/2 + (1-k) i(k-1-i); borderless communities, 0; singleton
+    communities, exact BC; a class of two vertices, the oracle."""
+    tol = 1e-12 if prec == F.PREC_F64 else 1e-6
+    for k in (3, 8, 40):
+        N = k + 1
+        s = np.arange(N)
+        out = F.clustered_bc(s, (s + 1) % N, np.ones(N), [1], partition=[0] * k + [1], form=form, prec=prec)
+        i = np.arange(k)
+        expect = np.full(N, (N - 1) * (N - 2) / 2.0)
+        expect[:k] += (1 - k) * i * (k - 1 - i)
+        assert sorted(out["pivots"].tolist()) == list(range(N)) and (out["weights"] == 1).all()
+        assert np.allclose(out["bc"], expect, rtol=tol, atol=tol * np.abs(expect).max()), (k, out["bc"], expect)
+    arcs = []
+    for base, k in ((0, 5), (5, 4), (9, 6)):
+        arcs += [(base + j, base + (j + 1) % k, 1 + j % 3) for j in range(k)]
+    a = np.array(arcs)
+    out = F.clustered_bc(a[:, 0], a[:, 1], a[:, 2], [1], partition=np.repeat([0, 1, 2], [5, 4, 6]), form=form,
+                         prec=prec)
+    assert len(out["pivots"]) == 0 and not out["bc"].any()
+    s, d, w = random_strong_arcs(3, 9)
+    og = O.Graph.from_arcs(s, d, w)
+    out = F.clustered_bc(s, d, w, [1], partition=np.arange(9), form=form, prec=prec)
+    ex = og.exact_bc()
+    assert np.allclose(out["bc"], ex, rtol=tol, atol=tol * ex.max())
+    s, d = np.array([0, 0, 1, 2, 3, 4]), np.array([1, 2, 3, 3, 4, 0])
+    w = np.ones(6)
+    part = np.array([0, 0, 0, 0, 1])
+    out = F.clustered_bc(s, d, w, [1], partition=part, form=form, prec=prec)
+    ref = O.clustered_bc_partition(O.Graph.from_arcs(s, d, w), part)
+    assert out["pivots"].tolist() == [0, 1, 3, 4] and out["weights"].tolist() == [1, 2, 1, 1]
+    assert np.array_equal(out["intra"], ref["intra"])
+    assert np.allclose(out["bc"], ref["bc"], rtol=tol, atol=tol * np.abs(ref["bc"]).max())
+
+
+def test_partition_entry_matches_louvain_run(gpu_ctx):
+    """cfg0 with the reference's Louvain partition handed in equals the run
+    that computes it (same communities, pivots and BC)."""
+    n, (s, d, w) = F.generate_config(0)
+    r = F.clustered_bc(s, d, w, [1, 2, 3, 4], 0.01, kfrac=0.5)
+    p = F.clustered_bc(s, d, w, [], partition=r["comm"], kfrac=0.5)
+    assert np.array_equal(p["comm"], r["comm"]) and np.array_equal(p["pivots"], r["pivots"])
+    assert np.array_equal(p["intra"], r["intra"]) and np.array_equal(p["weights"], r["weights"])
+    assert np.allclose(p["bc"], r["bc"], rtol=1e-12, atol=1e-9)
