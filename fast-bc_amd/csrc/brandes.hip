@@ -1269,6 +1269,10 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];
     const uint32_t r0 = t.rank(), R = t.size();
+    if (src >= V.n) {   /* the same for the whole team: it skips the task together */
+        if (t.leader()) atomicOr(p.err, ERR_SOURCE);
+        return;
+    }
 
     const uint64_t t0 = wall_clock64();
     phase_distances(V, S, t, src, p.delta, p.sort_cap, p.err, ct);

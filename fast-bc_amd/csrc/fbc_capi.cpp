@@ -305,6 +305,7 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
         ctx->last_counters[i] = (unsigned long long)((double)ctx->last_counters[i] * 1e6 / ctx->wall_khz);
     ctx->last_ms = ms;
     ctx->last_te = (double)te;
+    if (err & fbcdev::ERR_SOURCE) return fail(ctx, FBC_E_ARG, "source out of range");
     if (err & fbcdev::ERR_SIGMA)
         return fail(ctx, FBC_E_SIGMA_OVERFLOW, "shortest-path count exceeds 2^64-1");
     if (err & fbcdev::ERR_DIST) return fail(ctx, FBC_E_DIST_OVERFLOW, "path length exceeds 2^32-2");
@@ -892,7 +893,7 @@ int fbc_clusters_create(fbc_ctx *ctx, const fbc_graph *g, uint32_t ncl, const ui
 
 int fbc_clusters_borders(const fbc_clusters *cl, uint32_t *borders)
 {
-    if (!cl || !borders) return FBC_E_ARG;
+    if (!cl || (cl->bptr[cl->ncl] && !borders)) return FBC_E_ARG;
     for (uint32_t c = 0; c < cl->ncl; ++c)
         for (uint32_t j = cl->bptr[c]; j < cl->bptr[c + 1]; ++j)
             borders[j] = cl->gid[cl->voff[c] + cl->blocal[j]];

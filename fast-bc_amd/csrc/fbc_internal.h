@@ -32,7 +32,8 @@ enum Mode : int {
 enum ErrBits : uint32_t {
     ERR_DIST = 1u,          /* path length beyond 2^32-2 */
     ERR_SIGMA = 2u,         /* sigma beyond 2^64-1 */
-    ERR_SIGMA_INEXACT = 4u  /* sigma >= 2^53: the reference's FP64 sigma is inexact */
+    ERR_SIGMA_INEXACT = 4u, /* sigma >= 2^53: the reference's FP64 sigma is inexact */
+    ERR_SOURCE = 8u         /* a task's source id is not a vertex of its view */
 };
 
 /* Everything a launch needs, passed by value. */

@@ -31,7 +31,7 @@ EXPORTS = (
 
 # Every symbol include/fbc_host.h declares.
 HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_clustered_bc_partition",
-                "fbc_exact_bc_arcs", "fbc_host_release")
+                "fbc_exact_bc_arcs", "fbc_host_release", "fbc_host_last_error")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
                  "ns_dist", "ns_sigma", "ns_dep", "sigma_wide",
@@ -117,6 +117,8 @@ def lib():
         L.fbc_exact_bc_arcs.argtypes = [ctypes.c_int, ctypes.c_uint64, i32p, i32p, _f64p, ctypes.c_int, _f64p]
         L.fbc_host_release.argtypes = []
         L.fbc_host_release.restype = None
+        L.fbc_host_last_error.argtypes = []
+        L.fbc_host_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -394,7 +396,7 @@ def _arcs_i32(src, dst, w):
 
 def _host_check(rc, what):
     if rc < 0:
-        raise FbcError(rc, what)
+        raise FbcError(rc, f"{what}: {lib().fbc_host_last_error().decode()}")
     return rc
 
 

@@ -17,18 +17,25 @@ using namespace fbchost;
 
 namespace {
 
+/* message of the calling thread's last failed host entry point */
+thread_local std::string last_error;
+
 template <typename F>
 int guarded(F f)
 {
     try {
         return f();
-    } catch (const std::invalid_argument &) {
+    } catch (const std::invalid_argument &e) {
+        last_error = e.what();
         return FBC_E_ARG;
-    } catch (const std::out_of_range &) {
+    } catch (const std::out_of_range &e) {
+        last_error = e.what();
         return FBC_E_ARG;
     } catch (const std::bad_alloc &) {
+        last_error = "host allocation failed";
         return FBC_E_OOM;
-    } catch (const std::exception &) {
+    } catch (const std::exception &e) {
+        last_error = e.what();
         return FBC_E_HIP;
     }
 }
@@ -239,5 +246,7 @@ int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t 
         return 0;
     });
 }
+
+const char *fbc_host_last_error(void) { return last_error.c_str(); }
 
 } // extern "C"

@@ -75,6 +75,11 @@ int fbc_clustered_bc_partition(int device, uint64_t m, const int32_t *src, const
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,
                       const double *w, int prec, double *bc);
 
+/* Message of the calling thread's last failed call above (empty when none
+ * failed yet).  No counterpart in the reference, whose errors are
+ * exceptions. */
+const char *fbc_host_last_error(void);
+
 /* fbc_clustered_bc and fbc_exact_bc_arcs keep one device context per device
  * (configured from FBC_OPTIONS, re-created when it changes) and the graph of
  * the last arcs they were given, so that a repeated call on the same arcs

@@ -12,6 +12,27 @@ Fixtures (numpy .npz, no pickles):
   kmeans_cases.npz      PlusPlusKMeans on seeded random VertexInfo rows
   cfg0_ssb.npz          DijkstraSSBrandesBC summed over 16 weighted pivots
   graph_digests.json    sha256 of the generated cfg0..cfg3 edge lists
+
+Large-configuration fixtures (`--part cfg3`, `--part cfg2`, `--part cfg4`;
+the default `--part base` writes the ones above):
+  cfg3_exact.npz        ExactBrandesBC::computeBC on the R-MAT scale-15 graph
+                        (ref_driver_fast, OpenMP; thread count recorded)
+  cfg2_reference.npz    on the 1024x1024 road grid: the reference Louvain
+                        partition (seeds 1,2,3,4, precision 0.01); for its
+                        three smallest communities, DijkstraClusterEvaluator's
+                        borders and intra-cluster BC, and sha256 digests of
+                        the VertexInfo rows (uint32 lengths, int32 counts,
+                        row-major); PlusPlusKMeans run by the reference on
+                        those communities' exact classes (k = P/2) with the
+                        resulting pivots and weights; DijkstraSSBrandesBC
+                        dependency rows of four of the run's pivots as sha256
+                        digests of their FP64 bytes (plus their sums)
+  cfg2_oracle_pivots.npz  the oracle's pivots, weights and pivot clusters of
+                        the whole cfg2 selection (all 211 communities), and a
+                        digest of its intra-cluster BC vector
+  cfg4_reference.npz    on the 2048x2048 road grid: digest and community
+                        count of the reference Louvain partition; dependency
+                        digests of two sources
 """
 import hashlib
 import json
@@ -120,5 +141,150 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+def sha(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), np.uint8).copy()
+
+
+def ref_dep_rows(ref, graph, sources, tmp):
+    """DijkstraSSBrandesBC::singleSourceBrandes rows (ssb mode, one pivot of
+    weight 1 per run: the summed row is the dependency row itself)."""
+    procs, outs = [], []
+    for s in sources:
+        pp = os.path.join(tmp, f"src{s}.txt")
+        with open(pp, "w") as f:
+            f.write(f"{int(s)} 1\n")
+        op = os.path.join(tmp, f"dep{s}.txt")
+        outs.append(op)
+        procs.append(subprocess.Popen([ref, "ssb", graph, pp, op],
+                                      env=dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="256M")))
+    for p in procs:
+        assert p.wait() == 0
+    return [np.loadtxt(op, comments="#") for op in outs]
+
+
+def part_cfg3(tmp):
+    n, arcs = F.generate_config(3)
+    p = os.path.join(tmp, "cfg3.txt")
+    F.write_edge_list(n, arcs, p)
+    out = os.path.join(tmp, "exact3.txt")
+    thr = min(os.cpu_count() or 1, 8)
+    r = subprocess.run([REF + "_fast", "exact", p, out], env=dict(os.environ, OMP_NUM_THREADS=str(thr)),
+                       capture_output=True, text=True, check=True)
+    np.savez_compressed(os.path.join(HERE, "cfg3_exact.npz"), bc=np.loadtxt(out), threads=np.array([thr]))
+    print(r.stderr.strip())
+
+
+def part_cfg2(tmp):
+    n, arcs = F.generate_config(2)
+    gp = os.path.join(tmp, "cfg2.txt")
+    F.write_edge_list(n, arcs, gp)
+    txt = run([REF + "_fast", "louvain", gp, "0.01", "1", "2", "3", "4"], env={"OMP_NUM_THREADS": "4"})
+    comm = np.array(txt.split(), dtype=np.int64)
+    assert len(comm) == n and comm.max() < 65536
+    res = {"comm": comm.astype(np.uint16)}
+    k = int(comm.max()) + 1
+    s, d = arcs[0].astype(np.int64), arcs[1].astype(np.int64)
+    cross = comm[s] != comm[d]
+    isb = np.zeros(n, bool)
+    isb[s[cross]] = True
+    isb[d[cross]] = True
+    sizes = np.bincount(comm, minlength=k)
+    nb = np.bincount(comm[isb], minlength=k)
+    small = [int(c) for c in np.argsort(sizes, kind="stable") if nb[c] > 0][:3]
+    res["clusters"] = np.array(small, np.int32)
+    for i, c in enumerate(small):
+        verts = np.where(comm == c)[0]
+        vp = os.path.join(tmp, f"c{c}.txt")
+        np.savetxt(vp, verts, fmt="%d")
+        op = os.path.join(tmp, f"c{c}_out.txt")
+        run([REF + "_fast", "cluster", gp, vp, op])
+        lines = open(op).read().split("\n")
+        borders = np.array(lines[0].split()[1:], dtype=np.int32)
+        rows = [ln.split() for ln in lines[1:] if ln]
+        assert [int(r[0]) for r in rows] == verts.tolist()
+        L = np.array([[float(x) for x in r[2::2]] for r in rows])
+        C = np.array([[int(x) for x in r[3::2]] for r in rows], dtype=np.int32)
+        assert (L == np.floor(L)).all() and L.max() < 2 ** 32
+        res[f"borders{i}"] = borders
+        res[f"intra{i}"] = np.array([float(r[1]) for r in rows])
+        res[f"shape{i}"] = np.array(L.shape, np.int64)
+        res[f"len_sha{i}"] = sha(L.astype(np.uint32))
+        res[f"cnt_sha{i}"] = sha(C)
+        # exact classes (normalized rows, first appearance) as the selector
+        # forms them, then the reference's PlusPlusKMeans on them, k = P/2
+        Ln = L - L.min(axis=1, keepdims=True)
+        key = {}
+        reps, card = [], []
+        for r in range(len(verts)):
+            t = (tuple(C[r]), tuple(Ln[r]))
+            if t not in key:
+                key[t] = len(reps)
+                reps.append(r)
+                card.append(0)
+            card[key[t]] += 1
+        P = len(reps)
+        kk = max(int(P * 0.5), 1)
+        kp = os.path.join(tmp, f"km{c}.txt")
+        with open(kp, "w") as f:
+            f.write(f"{P} {L.shape[1]} {kk}\n")
+            for j, r in enumerate(reps):
+                f.write(str(card[j]) + " " + " ".join(f"{Ln[r, b]:.17g} {C[r, b]}" for b in range(L.shape[1])) + "\n")
+        ko = os.path.join(tmp, f"km{c}_out.txt")
+        run([REF + "_fast", "kmeans", kp, ko])
+        km = np.loadtxt(ko, dtype=np.int64).reshape(-1, 2)
+        res[f"classes{i}"] = np.array([P], np.int64)
+        res[f"km_cent{i}"] = km[:, 0].astype(np.int32)     # class indices, before the dedupe
+        res[f"km_w{i}"] = km[:, 1].astype(np.int32)
+    op = os.path.join(HERE, "cfg2_oracle_pivots.npz")
+    if os.path.exists(op):
+        piv = np.load(op)
+        srcs = []
+        for c in np.unique(piv["pivot_cluster"])[[0, 50, 100, 150]]:
+            srcs.append(int(piv["pivots"][piv["pivot_cluster"] == c][0]))
+        rows = ref_dep_rows(REF + "_fast", gp, srcs, tmp)
+        res["dep_src"] = np.array(srcs, np.uint32)
+        res["dep_sha"] = np.stack([sha(np.asarray(r, np.float64)) for r in rows])
+        res["dep_sum"] = np.array([float(np.sum(r)) for r in rows])
+    np.savez_compressed(os.path.join(HERE, "cfg2_reference.npz"), **res)
+
+
+def part_cfg2_oracle(tmp):
+    """The oracle's whole cfg2 selection (slow: every cluster source)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    n, (s, d, w) = F.generate_config(2)
+    og = O.Graph.from_arcs(s, d, w.astype(np.float64))
+    r = O.clustered_pivots(og, [1, 2, 3, 4], 0.01, kfrac=0.5, threads=min(os.cpu_count() or 1, 8))
+    np.savez_compressed(os.path.join(HERE, "cfg2_oracle_pivots.npz"), pivots=r["pivots"].astype(np.uint32),
+                        weights=r["weights"].astype(np.uint32), pivot_cluster=r["pivot_cluster"].astype(np.uint32),
+                        intra_sha=sha(r["intra"]), comm_sha=sha(r["comm"].astype(np.int32)))
+
+
+def part_cfg4(tmp):
+    n, arcs = F.generate_config(4)
+    gp = os.path.join(tmp, "cfg4.txt")
+    F.write_edge_list(n, arcs, gp)
+    txt = run([REF + "_fast", "louvain", gp, "0.01", "1", "2", "3", "4"], env={"OMP_NUM_THREADS": "4"})
+    comm = np.array(txt.split(), dtype=np.int32)
+    assert len(comm) == n
+    srcs = [0, 2 * 1048576 + 1024, n - 1]
+    rows = ref_dep_rows(REF + "_fast", gp, srcs, tmp)
+    np.savez_compressed(os.path.join(HERE, "cfg4_reference.npz"), comm_sha=sha(comm),
+                        ncomm=np.array([int(comm.max()) + 1]), dep_src=np.array(srcs, np.uint32),
+                        dep_sha=np.stack([sha(np.asarray(r, np.float64)) for r in rows]),
+                        dep_sum=np.array([float(np.sum(r)) for r in rows]))
+
+
 if __name__ == "__main__":
-    main()
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--part", default="base", choices=["base", "cfg3", "cfg2", "cfg2_oracle", "cfg4"])
+    a = ap.parse_args()
+    if a.part == "base":
+        main()
+    else:
+        if not os.path.exists(REF):
+            sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref, needs /root/reference)")
+        t = tempfile.mkdtemp(prefix="fbc_golden_")
+        {"cfg3": part_cfg3, "cfg2": part_cfg2, "cfg2_oracle": part_cfg2_oracle, "cfg4": part_cfg4}[a.part](t)
+        print("written", a.part)

@@ -131,9 +131,10 @@ def test_composition_closed_forms(gpu_ctx, form, prec):
     for base, k in ((0, 5), (5, 4), (9, 6)):
         arcs += [(base + j, base + (j + 1) % k, 1 + j % 3) for j in range(k)]
     a = np.array(arcs)
-    out = F.clustered_bc(a[:, 0], a[:, 1], a[:, 2], [1], partition=np.repeat([0, 1, 2], [5, 4, 6]), form=form,
-                         prec=prec)
-    assert len(out["pivots"]) == 0 and not out["bc"].any()
+    for kfrac in (0.0, 0.5):
+        out = F.clustered_bc(a[:, 0], a[:, 1], a[:, 2], [1], partition=np.repeat([0, 1, 2], [5, 4, 6]), form=form,
+                             prec=prec, kfrac=kfrac)
+        assert len(out["pivots"]) == 0 and not out["bc"].any()
     s, d, w = random_strong_arcs(3, 9)
     og = O.Graph.from_arcs(s, d, w)
     out = F.clustered_bc(s, d, w, [1], partition=np.arange(9), form=form, prec=prec)
