@@ -1482,6 +1482,21 @@ hipError_t launch_plain(int mode, const BrandesParams &p, uint32_t task, uint32_
     return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(WG), args, 0, stream);
 }
 
+/* ---- clearing control words (a kernel, not a memset node) ------------------ */
+
+__global__ void k_clear_words(uint32_t *p, uint32_t nwords)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) p[i] = 0u;
+}
+
+hipError_t launch_clear_words(uint32_t *p, uint32_t nwords, hipStream_t stream)
+{
+    if (!nwords) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_words, dim3((nwords + 255) / 256), dim3(256), 0, stream, p, nwords);
+    return hipGetLastError();
+}
+
 /* ---- cluster reduction and pivot subtraction ------------------------------- */
 
 __global__ void k_intra_reduce(BrandesParams p, uint64_t vbegin, uint64_t vcount,

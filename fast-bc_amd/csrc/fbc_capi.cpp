@@ -14,12 +14,29 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
 #include <vector>
 
 using fbcdev::BrandesParams;
+
+/* Device memory of fbc_dep_batch_noalloc, the entry point for callers that
+ * cannot allocate: NOALLOC_CTL_BYTES of control words (work-queue head, error
+ * bits, traversed edges, work counters), then the slots of the sources in
+ * flight.  Every GPU the library runs on holds this array from the
+ * library's load until the process ends.  Its name is a host-side stand-in:
+ * host code uses only the per-device address that fbc_create obtains with
+ * hipGetSymbolAddress. */
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr size_t NOALLOC_CTL_BYTES = 256;
+__device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex noalloc_mu;
+std::vector<uint8_t *> noalloc_ptr;   /* by device: the array's address there */
+}
 
 struct fbc_ctx {
     int device = 0;
@@ -46,6 +63,7 @@ struct fbc_ctx {
     double last_ms = 0, last_te = 0;
     unsigned long long last_counters[fbcdev::STAT_COUNT] = {};
     double wall_khz = 100000.0;   /* device wall clock (wall_clock64) rate */
+    uint8_t *noalloc = nullptr;   /* this device's address of 'scratch' */
 };
 
 struct fbc_graph {
@@ -342,6 +360,21 @@ int fbc_create(int device, fbc_ctx **out)
         dalloc(&ctx->d_grid_ctl, 16) != hipSuccess || hipMemset(ctx->d_grid_ctl, 0, 16 * sizeof(uint32_t)) != hipSuccess) {
         fbc_destroy(ctx);
         return FBC_E_HIP;
+    }
+    {
+        /* once per device, with the device current: the address of the
+         * allocation-free entry point's array on this device */
+        std::lock_guard<std::mutex> lk(noalloc_mu);
+        if (noalloc_ptr.size() < (size_t)ndev) noalloc_ptr.resize((size_t)ndev, nullptr);
+        if (!noalloc_ptr[(size_t)device]) {
+            void *ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                fbc_destroy(ctx);
+                return FBC_E_HIP;
+            }
+            noalloc_ptr[(size_t)device] = static_cast<uint8_t *>(ptr);
+        }
+        ctx->noalloc = noalloc_ptr[(size_t)device];
     }
     *out = ctx;
     return 0;
@@ -699,6 +732,72 @@ int fbc_dep_batch(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_
     if (!ctx || !g || (nsrc && (!src || !dep))) return fail(ctx, FBC_E_ARG, "null argument");
     if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
     return row_mode(ctx, g, fbcdev::MODE_DEP, prec, src, nsrc, nullptr, nullptr, dep);
+}
+
+int fbc_dep_batch_noalloc(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *d_src, uint32_t nsrc, int prec,
+                          double *d_dep)
+{
+    if (!ctx || !g || (nsrc && (!d_src || !d_dep))) return fail(ctx, FBC_E_ARG, "null argument");
+    if (!valid_prec(prec)) return fail(ctx, FBC_E_ARG, "unknown precision");
+    if (!ctx->noalloc) return fail(ctx, FBC_E_HIP, "static scratch not resolved on this device");
+    ctx->last_ms = 0;
+    ctx->last_te = 0;
+    ctx->plain_sources = ctx->plain_fallback_sources = ctx->plain_refusal = 0;
+    if (nsrc == 0) return 0;
+    /* slots: the batched kernel's team for this graph size, as many whole
+     * workgroups' worth as the array holds (and the device keeps busy) */
+    const uint64_t sb = fbcdev::slot_bytes(g->n);
+    const int kind = fbcdev::team_kind(g->n, ctx->half_max);
+    const uint64_t unit = kind == fbcdev::TEAM_HALF ? fbcdev::HALVES : kind == fbcdev::TEAM_WAVE ? fbcdev::WAVES : 1;
+    const uint64_t per_cu = kind == fbcdev::TEAM_HALF ? 64 : kind == fbcdev::TEAM_WAVE ? 32 : 4;
+    const uint64_t fit = (NOALLOC_SCRATCH_BYTES - NOALLOC_CTL_BYTES) / (sb * unit) * unit;
+    if (fit == 0) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "graph of %u vertices: one source needs %llu bytes of scratch (%llu with its team), beyond the "
+                      "%zu MiB static array of the allocation-free entry point",
+                      g->n, (unsigned long long)sb, (unsigned long long)(sb * unit), NOALLOC_SCRATCH_BYTES >> 20);
+        return fail(ctx, FBC_E_UNSUPPORTED, msg);
+    }
+    uint64_t slots = std::min<uint64_t>(nsrc, (uint64_t)ctx->cu_count * per_cu);
+    slots = std::min(fit, (slots + unit - 1) / unit * unit);
+    uint8_t *const base = ctx->noalloc;
+    BrandesParams p;
+    base_params(p, g);
+    p.ntasks = nsrc;
+    p.task_src = d_src;
+    p.out_dep = d_dep;
+    p.scratch = base + NOALLOC_CTL_BYTES;
+    p.slot_bytes = sb;
+    p.nslots = (uint32_t)slots;
+    p.half_max = ctx->half_max;
+    p.delta = auto_delta(ctx, g);
+    p.sort_cap = ctx->sort_cap;
+    p.queue_head = reinterpret_cast<uint32_t *>(base);
+    p.err = reinterpret_cast<uint32_t *>(base + 4);
+    p.te = reinterpret_cast<unsigned long long *>(base + 8);
+    p.stats = reinterpret_cast<unsigned long long *>(base + 16);
+    static_assert(16 + 8 * fbcdev::STAT_COUNT <= NOALLOC_CTL_BYTES, "control words");
+    /* the last call's control words are still there: clear them */
+    HIP_TRY(ctx, fbcdev::launch_clear_words(reinterpret_cast<uint32_t *>(base), NOALLOC_CTL_BYTES / 4, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, fbcdev::launch_brandes(fbcdev::MODE_DEP, prec, true, p, (uint32_t)slots, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long ctl[2 + fbcdev::STAT_COUNT] = {};
+    HIP_TRY(ctx, hipMemcpy(ctl, base, sizeof(ctl), hipMemcpyDeviceToHost));
+    const uint32_t err = (uint32_t)(ctl[0] >> 32);
+    for (int i = 0; i < fbcdev::STAT_COUNT; ++i) ctx->last_counters[i] = ctl[2 + i];
+    for (int i = fbcdev::STAT_T_DIST; i <= fbcdev::STAT_T_DEP; ++i)
+        ctx->last_counters[i] = (unsigned long long)((double)ctx->last_counters[i] * 1e6 / ctx->wall_khz);
+    ctx->last_ms = ms;
+    ctx->last_te = (double)ctl[1];
+    if (err & fbcdev::ERR_SOURCE) return fail(ctx, FBC_E_ARG, "source out of range");
+    if (err & fbcdev::ERR_SIGMA) return fail(ctx, FBC_E_SIGMA_OVERFLOW, "shortest-path count exceeds 2^64-1");
+    if (err & fbcdev::ERR_DIST) return fail(ctx, FBC_E_DIST_OVERFLOW, "path length exceeds 2^32-2");
+    return 0;
 }
 
 int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec, double *dep)

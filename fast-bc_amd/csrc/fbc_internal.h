@@ -129,6 +129,10 @@ inline int team_kind(uint32_t nmax, uint32_t half_max)
  * original numbering). */
 hipError_t launch_gather_add(uint32_t n, const uint32_t *perm, const double *src, double *dst, hipStream_t stream);
 
+/* p[0, nwords) = 0 by a kernel (safe inside captured graphs, unlike a
+ * memset node). */
+hipError_t launch_clear_words(uint32_t *p, uint32_t nwords, hipStream_t stream);
+
 /* Bytes of one slot for views of at most nmax vertices. */
 uint64_t slot_bytes(uint32_t nmax);
 

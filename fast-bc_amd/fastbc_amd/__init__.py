@@ -26,7 +26,7 @@ EXPORTS = (
     "fbc_clusters_free", "fbc_cluster_eval", "fbc_pivot_phase", "fbc_last_stats",
     "fbc_last_counters", "fbc_kmeans_batch", "fbc_pivot_phase_plain", "fbc_set_option",
     "fbc_cluster_eval_resident", "fbc_cluster_rows", "fbc_cluster_classes", "fbc_cluster_kmeans",
-    "fbc_cluster_class_features",
+    "fbc_cluster_class_features", "fbc_dep_batch_noalloc",
 )
 
 # Every symbol include/fbc_host.h declares.
@@ -77,6 +77,7 @@ def lib():
         L.fbc_sssp_sigma.argtypes = [_vp, _vp, _u32p, ctypes.c_uint32, _u64p, _u64p]
         L.fbc_single_source_dep.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _f64p]
         L.fbc_dep_batch.argtypes = [_vp, _vp, _u32p, ctypes.c_uint32, ctypes.c_int, _f64p]
+        L.fbc_dep_batch_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]
         L.fbc_exact_bc.argtypes = [_vp, _vp, ctypes.c_int, _f64p]
         L.fbc_clusters_create.argtypes = [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp),
                                           _u32p]

@@ -113,6 +113,22 @@ int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int pr
 int fbc_dep_batch(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32_t nsrc,
                   int prec, double *dep);
 
+/* fbc_dep_batch for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  d_src (nsrc source ids) and d_dep
+ * (nsrc x n doubles, row-major, every entry written) are device pointers the
+ * caller owns; the graph is one fbc_graph_upload made earlier.  The call
+ * allocates nothing: its per-source state lives in a 64 MiB array that each
+ * GPU running the library holds until the process ends, so a graph whose
+ * one source (a wavefront team's four, or a half-wavefront team's eight, for
+ * graphs of at most 65,536 or 2,048 vertices) needs more than that --
+ * about 40 bytes per vertex, beyond roughly 1.6 million vertices -- is
+ * refused with FBC_E_UNSUPPORTED.  Values equal fbc_dep_batch's bit for bit.
+ * Calls that overlap on one GPU (several contexts or threads) share the
+ * array and race. */
+int fbc_dep_batch_noalloc(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *d_src, uint32_t nsrc,
+                          int prec, double *d_dep);
+
 /* ExactBrandesBC::computeBC (brandes/ExactBrandesBC.h:46-95): bc[n] = sum over
  * all sources s of dep_s(v), v != s.  All sources in one persistent launch. */
 int fbc_exact_bc(fbc_ctx *ctx, const fbc_graph *g, int prec, double *bc);
