@@ -1,30 +1,37 @@
 /*
- * Persistent single-source Brandes kernel for gfx950.
+ * Persistent single-source Brandes kernels for gfx950.
  *
- * One workgroup (4 wavefronts) owns one source at a time and pulls the next
- * (view, source) task from a device work queue, so every source of a run --
- * all vertices (--exact), all cluster vertices, or all pivots -- goes through
- * one launch.  Per source:
+ * A team -- one workgroup, one wavefront or one half wavefront, by the size
+ * of the view -- owns one source at a time and pulls the next (view, source)
+ * task from a device work queue, so every source of a run -- all vertices
+ * (--exact), all cluster vertices, or all pivots -- goes through one launch.
+ * Per source:
  *
- *   1. distances: near-far delta-stepping.  Relaxations use 32-bit atomicMin
- *      on the tentative distance; vertices whose distance improves below the
- *      window limit are deduplicated per round with an atomicMax round stamp
- *      and appended to the next near queue, the others to the far queue.
- *   2. sigma: a Kahn sweep over the shortest-path DAG (arc u->w with
- *      d(u) + w = d(w)).  In-degrees are pulled over the reverse CSR; sigma is
- *      pushed with 64-bit integer atomics (exact, overflow checked) and every
- *      vertex is appended to the level list when its last DAG predecessor is
- *      done.  The level list is a topological order of the DAG.
+ *   1. distances: near-far delta-stepping.  Each vertex has one 64-bit key,
+ *      tentative distance (high 32 bits) and a tag (low 32): the near round
+ *      it was last queued for, or the window whose far list holds it.  A
+ *      relaxation is one 64-bit atomicMin on the key; the key it returns
+ *      tells whether the vertex is queued for the next round (or on this
+ *      window's far list) already, so each is appended at most once.
+ *   2. sigma: a sweep in vertex order counts each vertex's shortest-path DAG
+ *      in-degree over the reverse CSR and records its tight-arc mask; a Kahn
+ *      sweep then pushes sigma along the masked arcs with exact 64-bit
+ *      integer atomics (the in-degree packed above a 48-bit sigma, or, after
+ *      an overflow, a wide form with an overflow check), appending every
+ *      vertex to the level list when its last DAG predecessor is done.  The
+ *      level list is a topological order of the DAG.
  *   3. dependencies: the levels in reverse; every vertex pulls
- *      sigma(v) / sigma(w) * (1 + delta(w)) from its DAG successors.  In
- *      'ordered' mode the successors are summed in descending (d(w), w), the
- *      order in which the reference's backward stack visits them
+ *      sigma(v) / sigma(w) * (1 + delta(w)) from its masked DAG successors.
+ *      In 'ordered' mode the successors are summed in descending (d(w), w),
+ *      the order in which the reference's backward stack visits them
  *      (DijkstraSSBrandesBC.h:61-78), so the FP64 result equals the
- *      reference's bit for bit.
+ *      reference's bit for bit.  The mode's output follows in one pass in
+ *      vertex order.
  *
  * Views are CSRs in HBM (the whole graph, or one cluster's local CSR whose
- * local ids follow ascending global ids).  Per-slot state (distance, stamps,
- * queues, sigma, delta) lives in a private scratch region of the slot.
+ * local ids follow ascending global ids), with 16-byte short rows beside them
+ * for the distance phase.  Per-slot state (keys, queues, sigma/mask/delta
+ * pairs) lives in a private scratch region of the slot: 40 bytes per vertex.
  */
 #include "fbc_internal.h"
 
@@ -199,11 +206,16 @@ __device__ __forceinline__ F2 f2_div(F2 x, F2 y)
     float q2 = r.hi / y.hi;
     return quick_two_sum(q1, q2);
 }
+/* v = x1 * 2^24 + x0 with x0, x1 < 2^24: both are exact in FP32 and their
+ * two-sum is v exactly, for every v below 2^48 (the packed sigma form's
+ * range); a larger v (the wide form) rounds its high part to 24 bits, the
+ * pair's own precision. */
 __device__ __forceinline__ F2 f2_from_u64(uint64_t v)
 {
-    float hi = (float)v;
-    double rem = (double)v - (double)hi;   /* exact below 2^53, close above */
-    return {hi, (float)rem};
+    const float x0 = (float)(uint32_t)(v & 0xFFFFFFu);
+    const uint64_t h = v >> 24;
+    const float x1 = (h >> 32 ? (float)(uint32_t)(h >> 16) * 65536.0f : (float)(uint32_t)h) * 16777216.0f;
+    return two_sum(x1, x0);
 }
 __device__ __forceinline__ uint64_t f2_pack(F2 x)
 {
@@ -1329,7 +1341,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
         const uint32_t task = sh.task;
         __syncthreads();
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
     }
     flush_counters(p, t, ct);
 }
@@ -1354,7 +1366,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
         task = __shfl(task, 0, 64);
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
     }
     flush_counters(p, t, ct);
 }
@@ -1377,7 +1389,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
         task = __shfl(task, (int)(threadIdx.x & 32), 64);
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, task, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
     }
     flush_counters(p, t, ct);
 }

@@ -64,6 +64,17 @@ struct fbc_ctx {
     unsigned long long last_counters[fbcdev::STAT_COUNT] = {};
     double wall_khz = 100000.0;   /* device wall clock (wall_clock64) rate */
     uint8_t *noalloc = nullptr;   /* this device's address of 'scratch' */
+    uint32_t autotune = 1;        /* tune delta / team form on large batched runs */
+    /* the last batched run: its delta and half-team limit, and whether they
+     * were measured by that run (1), taken from the graph's cache (2) or
+     * fixed (0) */
+    uint32_t last_delta = 0, last_half_max = 0, last_tuned = 0;
+};
+
+/* An autotuned launch configuration. */
+struct TuneChoice {
+    bool set = false;
+    uint32_t delta = 0, half_max = 0;
 };
 
 struct fbc_graph {
@@ -84,6 +95,8 @@ struct fbc_graph {
     uint2 *d_barcs = nullptr, *d_brarcs = nullptr;
     uint4 *d_bell = nullptr;
     uint32_t *d_brp = nullptr, *d_brrp = nullptr;
+    /* the autotuner's choices for this graph: pivot phase, exact, clusters */
+    TuneChoice tuned[3];
 };
 
 struct fbc_clusters {
@@ -174,12 +187,12 @@ uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
 
 /* Make room for 'slots' slots of views up to nmax vertices; returns the slot
  * count actually available. */
-int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out)
+int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out, uint32_t half_max)
 {
     const uint64_t sb = fbcdev::slot_bytes(nmax);
     /* whole-graph slots: 4 per CU (throughput levels off near 3); a
      * wavefront-per-source workgroup holds 4 slots, 8 such workgroups per CU */
-    const int kind = fbcdev::team_kind(nmax, ctx->half_max);
+    const int kind = fbcdev::team_kind(nmax, half_max);
     const uint32_t per_cu = kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 4u;
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
@@ -232,6 +245,7 @@ void base_params(BrandesParams &p, const fbc_graph *g)
     p.rrp = g->d_rrp;
     p.n = g->n;
     p.nmax = g->n;
+    p.task_stride = 1;
 }
 
 /* Statuses with which the runtime refuses a cooperative launch for want of
@@ -242,36 +256,171 @@ bool refused_cooperative(hipError_t e)
            e == hipErrorLaunchOutOfResources || e == hipErrorOutOfMemory;
 }
 
-/* Launch one run over p.ntasks tasks and wait for it: the persistent batched
- * kernel, or (plain) one cooperative grid launch per task. */
-int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p,
-        bool plain = false)
+/* One launch of the batched kernel over q's tasks with q's delta and half-team
+ * limit, waited for.  Its time, traversed edges and work counters add to the
+ * context's figures of the current run; its error bits are or-ed into *err. */
+int timed_launch(fbc_ctx *ctx, int mode, int prec, bool ordered, BrandesParams q, uint32_t *err, double *ms_out,
+                 double *te_out)
 {
-    if (p.ntasks == 0) {
-        ctx->last_ms = 0;
-        ctx->last_te = 0;
-        return 0;
-    }
-    ctx->plain_sources = ctx->plain_fallback_sources = ctx->plain_refusal = 0;
     uint32_t slots = 0;
-    int rc = ensure_scratch(ctx, p.nmax, plain ? 1 : p.ntasks, &slots);
+    int rc = ensure_scratch(ctx, q.nmax, q.ntasks, &slots, q.half_max);
     if (rc) return rc;
-    p.scratch = ctx->scratch;
-    p.slot_bytes = fbcdev::slot_bytes(p.nmax);
-    p.nslots = slots;
-    p.half_max = ctx->half_max;
-    p.delta = auto_delta(ctx, g);
-    p.sort_cap = ctx->sort_cap;
-    p.queue_head = ctx->d_queue;
-    p.err = ctx->d_err;
-    p.te = ctx->d_te;
-    p.stats = ctx->d_stats;
+    q.scratch = ctx->scratch;
+    q.slot_bytes = fbcdev::slot_bytes(q.nmax);
+    q.nslots = slots;
+    q.queue_head = ctx->d_queue;
+    q.err = ctx->d_err;
+    q.te = ctx->d_te;
+    q.stats = ctx->d_stats;
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(ctx->last_counters), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_queue, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_te, 0, sizeof(unsigned long long), ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (plain) {
+    HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, q, slots, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    uint32_t e = 0;
+    unsigned long long te = 0, c[fbcdev::STAT_COUNT] = {};
+    HIP_TRY(ctx, hipMemcpy(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(c, ctx->d_stats, sizeof(c), hipMemcpyDeviceToHost));
+    for (int i = fbcdev::STAT_T_DIST; i <= fbcdev::STAT_T_DEP; ++i)   /* wall-clock ticks -> ns */
+        c[i] = (unsigned long long)((double)c[i] * 1e6 / ctx->wall_khz);
+    for (int i = 0; i < fbcdev::STAT_COUNT; ++i) ctx->last_counters[i] += c[i];
+    ctx->last_ms += ms;
+    ctx->last_te += (double)te;
+    *err |= e;
+    if (ms_out) *ms_out = ms;
+    if (te_out) *te_out = (double)te;
+    return 0;
+}
+
+/* The autotuner's slot for a mode (-1: the mode is not tuned). */
+int tune_slot(int mode)
+{
+    return mode == fbcdev::MODE_PIVOT ? 0 : mode == fbcdev::MODE_EXACT ? 1 : mode == fbcdev::MODE_CLUSTER ? 2 : -1;
+}
+
+/* The batched form of a run.  Large runs of the additive modes (pivot phase,
+ * exact BC, cluster evaluation: every task's output is its own) on a graph
+ * with no tuned configuration yet tune one on their own tasks: the first
+ * K * T tasks form K interleaved trial sets (task k, k + K, k + 2K, ...), one
+ * per candidate (delta of 4, 2 and 8 mean arc weights; for views small
+ * enough for half-wavefront teams, also whole wavefronts), T = two tasks per
+ * slot, each set one full launch; the candidate with the most traversed
+ * edges per millisecond runs the remaining tasks and is kept for the graph.
+ * No task is run twice. */
+int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p, uint32_t *err)
+{
+    TuneChoice def;
+    def.delta = auto_delta(ctx, g);
+    def.half_max = ctx->half_max;
+    const int ts = tune_slot(mode);
+    const bool tunable = ctx->autotune && !ctx->delta_override && ts >= 0;
+    ctx->last_tuned = 0;
+    TuneChoice use = def;
+    if (tunable && g->tuned[ts].set) {
+        use = g->tuned[ts];
+        ctx->last_tuned = 2;
+    } else if (tunable) {
+        std::vector<TuneChoice> cand;
+        for (double m : {4.0, 2.0, 8.0}) {
+            TuneChoice c;
+            c.delta = (uint32_t)std::min(std::max(g->meanw * m, 1.0), 1e9);
+            c.half_max = def.half_max;
+            cand.push_back(c);
+        }
+        if (fbcdev::team_kind(p.nmax, def.half_max) == fbcdev::TEAM_HALF) {
+            TuneChoice c;
+            c.delta = cand[0].delta;
+            c.half_max = 0;
+            cand.push_back(c);
+        }
+        const uint32_t K = (uint32_t)cand.size();
+        uint32_t slots = 0;
+        int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max);
+        if (rc) return rc;
+        const uint64_t T = 2ull * slots;
+        if ((uint64_t)p.ntasks >= 2 * K * T) {
+            double best = -1;
+            for (uint32_t k = 0; k < K; ++k) {
+                BrandesParams q = p;
+                q.task_base = k;
+                q.task_stride = K;
+                q.ntasks = (uint32_t)T;
+                q.delta = cand[k].delta;
+                q.half_max = cand[k].half_max;
+                double ms = 0, te = 0;
+                rc = timed_launch(ctx, mode, prec, ordered, q, err, &ms, &te);
+                if (rc) return rc;
+                const double rate = te / std::max(ms, 1e-3);
+                if (rate > best) {
+                    best = rate;
+                    use = cand[k];
+                }
+            }
+            use.set = true;
+            g->tuned[ts] = use;
+            ctx->last_tuned = 1;
+            BrandesParams q = p;
+            q.task_base = (uint32_t)(K * T);
+            q.ntasks = p.ntasks - (uint32_t)(K * T);
+            q.delta = use.delta;
+            q.half_max = use.half_max;
+            ctx->last_delta = use.delta;
+            ctx->last_half_max = use.half_max;
+            return q.ntasks ? timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr) : 0;
+        }
+    }
+    ctx->last_delta = use.delta;
+    ctx->last_half_max = use.half_max;
+    BrandesParams q = p;
+    q.delta = use.delta;
+    q.half_max = use.half_max;
+    return timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr);
+}
+
+/* Launch one run over p.ntasks tasks and wait for it: the persistent batched
+ * kernel (autotuned, run_batched), or (plain) one cooperative grid launch per
+ * task. */
+int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, BrandesParams &p,
+        bool plain = false)
+{
+    fbc_graph *g = const_cast<fbc_graph *>(cg);   /* the autotuner's cache */
+    ctx->last_ms = 0;
+    ctx->last_te = 0;
+    std::memset(ctx->last_counters, 0, sizeof(ctx->last_counters));
+    ctx->plain_sources = ctx->plain_fallback_sources = ctx->plain_refusal = 0;
+    if (p.ntasks == 0) return 0;
+    p.sort_cap = ctx->sort_cap;
+    uint32_t err = 0;
+    int rc = 0;
+    if (!plain) {
+        rc = run_batched(ctx, g, mode, prec, ordered, p, &err);
+        if (rc) return rc;
+    } else {
+        uint32_t slots = 0;
+        rc = ensure_scratch(ctx, p.nmax, 1, &slots, ctx->half_max);
+        if (rc) return rc;
+        p.scratch = ctx->scratch;
+        p.slot_bytes = fbcdev::slot_bytes(p.nmax);
+        p.nslots = slots;
+        p.half_max = ctx->half_max;
+        p.delta = auto_delta(ctx, g);
+        p.queue_head = ctx->d_queue;
+        p.err = ctx->d_err;
+        p.te = ctx->d_te;
+        p.stats = ctx->d_stats;
+        ctx->last_delta = p.delta;
+        ctx->last_half_max = p.half_max;
+        ctx->last_tuned = 0;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(ctx->last_counters), ctx->stream));
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_te, 0, sizeof(unsigned long long), ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         uint32_t blocks = 0;
         HIP_TRY(ctx, fbcdev::plain_max_blocks(mode, ctx->cu_count, &blocks));
         const uint32_t want = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count;
@@ -294,35 +443,28 @@ int run(fbc_ctx *ctx, const fbc_graph *g, int mode, int prec, bool ordered, Bran
             break;
         }
         ctx->plain_sources = t;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        unsigned long long te = 0;
+        HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(err), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(ctx->last_counters, ctx->d_stats, sizeof(ctx->last_counters), hipMemcpyDeviceToHost));
+        for (int i = fbcdev::STAT_T_DIST; i <= fbcdev::STAT_T_DEP; ++i)   /* wall-clock ticks -> ns */
+            ctx->last_counters[i] = (unsigned long long)((double)ctx->last_counters[i] * 1e6 / ctx->wall_khz);
+        ctx->last_ms = ms;
+        ctx->last_te = (double)te;
         if (t < p.ntasks) {
             /* The remaining sources run on the batched kernel, which computes
              * the same values (FP64, reference successor order); the count is
              * reported by fbc_last_counters. */
             ctx->plain_fallback_sources = p.ntasks - t;
             BrandesParams q = p;
-            uint32_t qslots = 0;
-            rc = ensure_scratch(ctx, q.nmax, q.ntasks, &qslots);
+            rc = timed_launch(ctx, mode, 0, true, q, &err, nullptr, nullptr);
             if (rc) return rc;
-            q.scratch = ctx->scratch;
-            q.nslots = qslots;
-            HIP_TRY(ctx, fbcdev::launch_brandes(mode, 0, true, q, qslots, ctx->stream));
         }
-    } else {
-        HIP_TRY(ctx, fbcdev::launch_brandes(mode, prec, ordered, p, slots, ctx->stream));
     }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    uint32_t err = 0;
-    unsigned long long te = 0;
-    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(err), hipMemcpyDeviceToHost));
-    HIP_TRY(ctx, hipMemcpy(&te, ctx->d_te, sizeof(te), hipMemcpyDeviceToHost));
-    HIP_TRY(ctx, hipMemcpy(ctx->last_counters, ctx->d_stats, sizeof(ctx->last_counters), hipMemcpyDeviceToHost));
-    for (int i = fbcdev::STAT_T_DIST; i <= fbcdev::STAT_T_DEP; ++i)   /* wall-clock ticks -> ns */
-        ctx->last_counters[i] = (unsigned long long)((double)ctx->last_counters[i] * 1e6 / ctx->wall_khz);
-    ctx->last_ms = ms;
-    ctx->last_te = (double)te;
     if (err & fbcdev::ERR_SOURCE) return fail(ctx, FBC_E_ARG, "source out of range");
     if (err & fbcdev::ERR_SIGMA)
         return fail(ctx, FBC_E_SIGMA_OVERFLOW, "shortest-path count exceeds 2^64-1");
@@ -416,6 +558,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
     else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
     else if (!std::strcmp(name, "half_team_max")) ctx->half_max = std::min<uint32_t>(v, fbcdev::WAVE_TEAM_MAX_N);
+    else if (!std::strcmp(name, "autotune")) ctx->autotune = v ? 1 : 0;
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
@@ -1734,7 +1877,8 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count)
 {
     if (!ctx || !counters || count < 0) return FBC_E_ARG;
-    const uint64_t host[] = {ctx->plain_sources, ctx->plain_fallback_sources, ctx->plain_refusal};
+    const uint64_t host[] = {ctx->plain_sources, ctx->plain_fallback_sources, ctx->plain_refusal,
+                             ctx->last_delta,    ctx->last_half_max,           ctx->last_tuned};
     constexpr int NH = (int)(sizeof(host) / sizeof(host[0]));
     int i = 0;
     for (; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];

@@ -56,8 +56,11 @@ struct BrandesParams {
     const uint64_t *cl_vioff; /* VertexInfo block offset per cluster */
     const uint64_t *cl_doff;  /* dependency-matrix offset per cluster (this chunk) */
 
-    /* tasks: source (local id in cluster mode), aux = cluster index or weight */
+    /* tasks: source (local id in cluster mode), aux = cluster index or weight;
+     * the batched kernels' queue index q runs task task_base + q * task_stride
+     * (the autotuner's interleaved trial sets; 0 and 1 otherwise) */
     uint32_t ntasks;
+    uint32_t task_base, task_stride;
     const uint32_t *task_src;
     const uint32_t *task_aux;
     uint32_t *queue_head;

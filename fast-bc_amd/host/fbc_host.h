@@ -105,6 +105,9 @@ struct PhaseTimes {
     size_t kmeans_gpu_sets = 0, kmeans_host_sets = 0;
     double cluster_te = 0, pivot_te = 0, cluster_kernel_ms = 0, pivot_kernel_ms = 0;
     size_t clusters = 0, bordered = 0, pivot_count = 0;
+    /* the autotuner's choice for the cluster and pivot launches: delta,
+     * half-team limit, and how chosen (fbc_last_counters' last three) */
+    uint64_t cluster_tuning[3] = {0, 0, 0}, pivot_tuning[3] = {0, 0, 0};
 };
 
 struct ClusteredOptions {

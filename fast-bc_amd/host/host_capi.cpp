@@ -53,6 +53,7 @@ struct Resident {
     std::vector<int32_t> src, dst;
     std::vector<double> w;
     std::shared_ptr<Graph> graph;
+    uint64_t tuning[6] = {0, 0, 0, 0, 0, 0};   /* of the last clustered run */
 };
 
 Resident &resident()
@@ -127,6 +128,8 @@ int clustered_entry(int device, uint64_t m, const int32_t *src, const int32_t *d
         if (std::getenv("FBC_PROGRESS"))   /* progress lines on stderr */
             opt.progress = [](int, const std::string &m) { std::fprintf(stderr, "[fbc] %s\n", m.c_str()); };
         ClusteredRun r = clustered_bc(dev, g, opt);
+        std::copy(r.t.cluster_tuning, r.t.cluster_tuning + 3, resident().tuning);
+        std::copy(r.t.pivot_tuning, r.t.pivot_tuning + 3, resident().tuning + 3);
         std::memcpy(bc, r.bc.data(), r.bc.size() * sizeof(double));
         if (comm_of)
             for (size_t i = 0; i < r.comm.size(); ++i) comm_of[i] = r.comm[i];
@@ -248,5 +251,13 @@ int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t 
 }
 
 const char *fbc_host_last_error(void) { return last_error.c_str(); }
+
+int fbc_host_last_tuning(uint64_t *out)
+{
+    if (!out) return FBC_E_ARG;
+    std::lock_guard<std::mutex> lk(resident().mu);
+    std::copy(resident().tuning, resident().tuning + 6, out);
+    return 0;
+}
 
 } // extern "C"

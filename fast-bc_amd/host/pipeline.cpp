@@ -337,6 +337,14 @@ std::vector<double> single_source_dep(Device &dev, const Graph &g, int src, int 
     return dep;
 }
 
+/* delta, half-team limit and how they were chosen, for the last launch */
+static void last_tuning(Device &dev, uint64_t out[3])
+{
+    uint64_t c[32] = {};
+    const int k = fbc_last_counters(dev.ctx(), c, 32);
+    for (int i = 0; i < 3; ++i) out[i] = k >= 17 ? c[14 + i] : 0;
+}
+
 ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &opt)
 {
     ClusteredRun run;
@@ -403,6 +411,7 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
     /* the VertexInfo rows stay on the device for the class selection */
     dev.check(fbc_cluster_eval_resident(dev.ctx(), cl, run.intra.data()));
     fbc_last_stats(dev.ctx(), &run.t.cluster_kernel_ms, &run.t.cluster_te);
+    last_tuning(dev, run.t.cluster_tuning);
     run.t.cluster = now() - t0;
     {
         char buf[200];
@@ -472,6 +481,7 @@ ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &o
                                           run.bc.data()));
             double ms = 0, te = 0;
             fbc_last_stats(dev.ctx(), &ms, &te);
+            if (p0 == 0) last_tuning(dev, run.t.pivot_tuning);
             run.t.pivot_kernel_ms += ms;
             run.t.pivot_te += te;
             if (np > chunk) report(1, "pivots done: " + std::to_string(p0 + k) + " / " + std::to_string(np));

@@ -81,8 +81,14 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * keeps; 0: all 64; a few bits force collisions, a test hook),
  * "half_team_max" (views of at most this many vertices run one source per
  * half wavefront, two sources sharing its instruction stream; default 2048,
- * 0: off).  No
- * counterpart in the reference. */
+ * 0: off), "autotune" (1, the default: a large batched run of the pivot
+ * phase, exact BC or the cluster evaluation on a graph with no tuned
+ * configuration yet runs its first tasks as interleaved trial sets, one per
+ * candidate -- delta of 4, 2 and 8 mean arc weights, and for views small
+ * enough for half-wavefront teams also whole wavefronts -- keeps the
+ * candidate with the most traversed edges per millisecond for the rest of
+ * the run and for later runs on the graph; an explicit "delta" turns it
+ * off; 0: off).  No counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
@@ -252,7 +258,10 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * (fbc_single_source_dep with FBC_PREC_F64, fbc_pivot_phase_plain), the
  * sources run by cooperative whole-GPU launches, the sources finished on the
  * batched kernel after a refused first launch, and the HIP status of that
- * refusal (0: none).  Fills at most 'count' entries, returns how many it
+ * refusal (0: none); then the delta and half-team limit of the last run's
+ * (main) launch and how they were chosen (0: fixed or set by option, 1:
+ * measured by the autotuner in this run, 2: the graph's tuned choice from an
+ * earlier run).  Fills at most 'count' entries, returns how many it
  * wrote.  Instrumentation only; no counterpart in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
 

@@ -75,6 +75,13 @@ int fbc_clustered_bc_partition(int device, uint64_t m, const int32_t *src, const
 int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t *dst,
                       const double *w, int prec, double *bc);
 
+/* The autotuner's choice in the last fbc_clustered_bc / _partition call:
+ * out[6] = {delta, half-team limit, how chosen} for the cluster launch, then
+ * the same for the pivot launch ("how": 0 fixed or set by option, 1
+ * measured in that call, 2 the graph's choice from an earlier call; see
+ * fbc_set_option "autotune").  No counterpart in the reference. */
+int fbc_host_last_tuning(uint64_t *out);
+
 /* Message of the calling thread's last failed call above (empty when none
  * failed yet).  No counterpart in the reference, whose errors are
  * exceptions. */

@@ -1,0 +1,64 @@
+"""The autotuner (fbc_set_option "autotune"): a large batched run tunes delta
+(and, for views small enough for half-wavefront teams, the team form) on
+interleaved trial sets of its own tasks, keeps the choice for the graph, and
+changes no result beyond the summation order.  Capping the slots makes
+cfg0's and cfg1's runs large enough to tune."""
+import os
+
+import numpy as np
+import pytest
+
+import fastbc_amd as F
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(os.cpu_count() or 1, 16)
+
+
+def test_autotune_exact_bc(gpu_ctx):
+    n, (s, d, w) = F.generate_config(0)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    mean = float(np.mean(w))
+    ref = O.Graph.from_arcs(s, d, w.astype(np.float64)).exact_bc(THREADS)
+    floor = 1e-12 * np.abs(ref).max()
+    gpu_ctx.set_tuning(0, 64)
+    try:
+        bc1 = g.exact_bc()
+        c1 = gpu_ctx.last_counters()
+        bc2 = g.exact_bc()
+        c2 = gpu_ctx.last_counters()
+        gpu_ctx.set_option("autotune", 0)
+        bc0 = g.exact_bc()
+        c0 = gpu_ctx.last_counters()
+    finally:
+        gpu_ctx.set_option("autotune", 1)
+        gpu_ctx.set_tuning(0, 0)
+    print(f"tuned delta {c1['delta']} (mean weight {mean:.1f}); cached {c2['delta']}; fixed {c0['delta']}", flush=True)
+    assert (c1["tuned"], c2["tuned"], c0["tuned"]) == (1, 2, 0)
+    assert c1["delta"] == c2["delta"] and c1["delta"] in {int(mean * m) for m in (2, 4, 8)}
+    assert c0["delta"] == int(mean * 4)
+    for bc in (bc1, bc2, bc0):
+        assert (np.abs(bc - ref) / np.maximum(np.abs(ref), floor)).max() <= 1e-9
+
+
+def test_autotune_cluster_eval_identical(gpu_ctx):
+    """Cluster evaluation tuned (half-wavefront or wavefront teams, three
+    deltas) equals the fixed configuration bit for bit: the intra-cluster
+    sums run in a fixed order whatever launched the rows."""
+    n, (s, d, w) = F.generate_config(1)
+    comm, _ = F.louvain(s, d, w, [1, 2, 3, 4], 0.01)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    cl = g.clusters(comm_of=comm)
+    gpu_ctx.set_tuning(0, 256)
+    try:
+        bc1, L1, C1 = cl.evaluate()
+        c1 = gpu_ctx.last_counters()
+        gpu_ctx.set_option("autotune", 0)
+        bc0, L0, C0 = cl.evaluate()
+    finally:
+        gpu_ctx.set_option("autotune", 1)
+        gpu_ctx.set_tuning(0, 0)
+    print(f"cluster views up to {int(np.bincount(comm).max())} vertices: tuned delta {c1['delta']}, "
+          f"half-team limit {c1['half_team_max']}", flush=True)
+    assert c1["tuned"] == 1
+    assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
