@@ -381,12 +381,23 @@ void kmeans(int k, int P, int nb, const double *L, const int *C, const int *weig
             std::fill(mcnt[c].begin(), mcnt[c].end(), 0.0);
             members[c].clear();
         }
+        /* A centroid that repeats an earlier one's point has the same
+         * distance to every point and a higher index, so the first strict
+         * minimum (PlusPlusKMeans.h:98-112) never picks it: scanning each
+         * point's first occurrence only gives the same assignment. */
+        std::vector<size_t> firsts;
+        {
+            std::set<int> seen;
+            for (size_t c = 0; c < K; ++c)
+                if (seen.insert(cent[c]).second) firsts.push_back(c);
+        }
         for (int v = 0; v < P; ++v) {
             const double *lv = L + (size_t)v * nb;
             const int *cv = C + (size_t)v * nb;
             int bestc = 0;
             double bestd = sqdist_points(L + (size_t)cent[0] * nb, C + (size_t)cent[0] * nb, lv, cv, nb);
-            for (size_t c = 1; c < K; ++c) {
+            for (size_t f = 1; f < firsts.size(); ++f) {
+                const size_t c = firsts[f];
                 double d = sqdist_points(L + (size_t)cent[c] * nb, C + (size_t)cent[c] * nb, lv, cv, nb);
                 if (d < bestd) { bestc = (int)c; bestd = d; }
             }

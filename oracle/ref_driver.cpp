@@ -13,6 +13,7 @@
  *   ssb     <graph> <pivots> <out>              DijkstraSSBrandesBC over "vertex weight" lines,
  *                                               OpenMP reduction as ClusteredBrandesBC.h:141-166
  *   cluster <graph> <vertices> <out>            DijkstraClusterEvaluator::evaluateCluster
+ *   clusters <graph> <partition> <c>...         the same over listed communities, timed
  *   louvain <graph> <precision> <seed>...       Partition::one_level / partition2graph
  *   kmeans  <rows> <out>                        PlusPlusKMeans::computeCentroids
  * Outputs are text with %.17g values.  Timings go to stderr.
@@ -180,6 +181,43 @@ static int mode_cluster(int argc, char **argv)
     return 0;
 }
 
+/* DijkstraClusterEvaluator over several clusters of one partition, the graph
+ * loaded once: partition = one community index per line (vertex order); the
+ * listed communities are evaluated in turn (ClusteredBrandesBC.h:91-121 runs
+ * them concurrently, each with its own evaluator call); prints per-cluster
+ * seconds and the total. */
+static int mode_clusters(int argc, char **argv)
+{
+    if (argc < 5) return 2;
+    auto g = load(argv[2]);
+    std::vector<int> comm;
+    {
+        std::ifstream in(argv[3]);
+        int c;
+        while (in >> c) comm.push_back(c);
+    }
+    if (comm.size() != g->vertices().size()) return 2;
+    double total = 0;
+    fastbc::brandes::DijkstraClusterEvaluator<int, double> ce;
+    for (int a = 4; a < argc; ++a) {
+        const int want = std::atoi(argv[a]);
+        std::vector<int> verts;
+        for (size_t v = 0; v < comm.size(); ++v)
+            if (comm[v] == want) verts.push_back((int)v);
+        double t0 = now_s();
+        auto sub = std::make_shared<DriverSubGraph>(verts, g);
+        std::vector<double> bc(g->vertices().size(), 0.0);
+        std::vector<std::shared_ptr<fastbc::brandes::VertexInfo<int, double>>> vi(g->vertices().size());
+        ce.evaluateCluster(bc, vi, sub);
+        const double dt = now_s() - t0;
+        total += dt;
+        std::printf("cluster %d vertices %zu edges %d borders %zu seconds %.6f\n", want, verts.size(), sub->edges(),
+                    sub->borders().size(), dt);
+    }
+    std::printf("total %.6f\n", total);
+    return 0;
+}
+
 /* The multi-instance level loop of LouvainGraphPartition::partitionGraph is
  * restated here (that header needs spdlog); Partition and LouvainGraph are
  * the reference's. */
@@ -266,7 +304,7 @@ static int mode_kmeans(int argc, char **argv)
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: ref_driver exact|ssb|cluster|louvain|kmeans ...\n");
+        std::fprintf(stderr, "usage: ref_driver exact|ssb|cluster|clusters|louvain|kmeans ...\n");
         return 2;
     }
     std::string m = argv[1];
@@ -274,6 +312,7 @@ int main(int argc, char **argv)
     if (m == "exact") rc = mode_exact(argc, argv);
     else if (m == "ssb") rc = mode_ssb(argc, argv);
     else if (m == "cluster") rc = mode_cluster(argc, argv);
+    else if (m == "clusters") rc = mode_clusters(argc, argv);
     else if (m == "louvain") rc = mode_louvain(argc, argv);
     else if (m == "kmeans") rc = mode_kmeans(argc, argv);
     if (rc == 2) std::fprintf(stderr, "bad arguments for mode %s\n", m.c_str());

@@ -37,6 +37,7 @@ the default `--part base` writes the ones above):
 import hashlib
 import json
 import os
+import resource
 import subprocess
 import sys
 import tempfile
@@ -155,8 +156,12 @@ def ref_dep_rows(ref, graph, sources, tmp):
             f.write(f"{int(s)} 1\n")
         op = os.path.join(tmp, f"dep{s}.txt")
         outs.append(op)
+        # the reference's array reduction over N doubles (ClusteredBrandesBC.h:143)
+        # keeps a private N-array on the thread's stack: 32 MB at 4M vertices
         procs.append(subprocess.Popen([ref, "ssb", graph, pp, op],
-                                      env=dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="256M")))
+                                      env=dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="256M"),
+                                      preexec_fn=lambda: resource.setrlimit(
+                                          resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))))
     for p in procs:
         assert p.wait() == 0
     return [np.loadtxt(op, comments="#") for op in outs]
