@@ -43,6 +43,12 @@ struct Graph {
  * reference cannot represent (arc into an id >= n, malformed line). */
 Graph load_edge_list(std::istream &in);
 Graph load_edge_list_file(const std::string &path);
+/* load_edge_list_file through a binary CSR cache at cache_path (no
+ * counterpart in the reference, which parses the text on every run): a cache
+ * written for the same text file -- same size, modification time and a hash
+ * of its first and last MiB -- is read instead of the text; otherwise the
+ * text is parsed and the cache (re)written.  *hit says which happened. */
+Graph load_edge_list_cached(const std::string &path, const std::string &cache_path, bool *hit = nullptr);
 /* Same semantics from arc arrays (file order). */
 Graph from_arcs(size_t m, const int *src, const int *dst, const double *w);
 /* The same with an explicit vertex count n (ids must be < n): the vertex set

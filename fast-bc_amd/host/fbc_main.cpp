@@ -16,6 +16,8 @@
  *         --form               batched | plain: all pivots in one launch (default),
  *                              or one whole-GPU launch per pivot (FP64)
  *         --full-output        also write full-precision, unclamped BC values
+ *         --csr-cache          binary CSR cache of the input (read when it
+ *                              matches the input file, else written)
  *
  * Option grammar (popl 1.2.0 as vendored by the reference, popl.hpp:919-1024):
  * '-x v', '-xv', '--long v', '--long=v'; unknown options are ignored; after
@@ -212,8 +214,10 @@ int main(int argc, char **argv)
         {0, "precision", "Dependency arithmetic: f64 | f32c (this build)", true},
         {0, "form", "Pivot phase: batched | plain (this build)", true},
         {0, "full-output", "Also write unclamped %.17g values to this file (this build)", true},
+        {0, "csr-cache", "Binary CSR cache of the input: read when it matches the input, else written (this build)",
+         true},
     };
-    std::string seedstr, outpath = "bc.txt", level = "info", fullout, precstr = "f64", formstr = "batched";
+    std::string seedstr, outpath = "bc.txt", level = "info", fullout, precstr = "f64", formstr = "batched", csrcache;
     int instances = 4, threads = 0, device = 0;
     double precision = 0.01, kfrac = 0;
     bool exact = false;
@@ -230,6 +234,7 @@ int main(int argc, char **argv)
         if (Opt *o = op.find(std::string("precision")); o->set) precstr = o->value;
         if (Opt *o = op.find(std::string("form")); o->set) formstr = o->value;
         if (Opt *o = op.find(std::string("full-output")); o->set) fullout = o->value;
+        if (Opt *o = op.find(std::string("csr-cache")); o->set) csrcache = o->value;
         exact = op.find(std::string("exact"))->set;
         if (precstr != "f64" && precstr != "f32c")
             throw InvalidOption("invalid argument for --precision: '" + precstr + "'");
@@ -295,7 +300,11 @@ int main(int argc, char **argv)
         return -1;
     }
     /* like the reference, a loader exception (weight <= 0) is not caught */
-    fbchost::Graph g = fbchost::load_edge_list(in);
+    bool cache_hit = false;
+    fbchost::Graph g = csrcache.empty() ? fbchost::load_edge_list(in)
+                                        : fbchost::load_edge_list_cached(input, csrcache, &cache_hit);
+    if (!csrcache.empty())
+        log(DEBUG, std::string(cache_hit ? "Graph read from the CSR cache " : "CSR cache written to ") + csrcache);
     log(INFO, "Loaded graph contains " + std::to_string(g.n) + " vertices and " + std::to_string(g.edges) +
                   " edges");
 

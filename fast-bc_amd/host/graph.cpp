@@ -26,6 +26,7 @@
 #include <iterator>
 #include <numeric>
 #include <stdexcept>
+#include <sys/stat.h>
 
 namespace fbchost {
 
@@ -264,6 +265,118 @@ Graph load_edge_list(std::istream &in)
 {
     std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
     return graph_from_text(text);
+}
+
+namespace {
+
+/* The text file a cache belongs to: size, modification time, and an FNV-1a
+ * hash of its first and last MiB. */
+struct TextPrint {
+    uint64_t size = 0;
+    int64_t mtime_ns = 0;
+    uint64_t hash = 0;
+    bool operator==(const TextPrint &o) const { return size == o.size && mtime_ns == o.mtime_ns && hash == o.hash; }
+};
+
+bool text_print(const std::string &path, TextPrint &tp)
+{
+    struct stat st;
+    if (::stat(path.c_str(), &st) != 0) return false;
+    tp.size = (uint64_t)st.st_size;
+    tp.mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000ll + st.st_mtim.tv_nsec;
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    uint64_t h = 1469598103934665603ull;
+    std::vector<unsigned char> buf(1 << 20);
+    auto mix = [&](size_t r) {
+        for (size_t i = 0; i < r; ++i) h = (h ^ buf[i]) * 1099511628211ull;
+    };
+    mix(std::fread(buf.data(), 1, buf.size(), f));
+    if (tp.size > buf.size() && std::fseek(f, -(long)buf.size(), SEEK_END) == 0) mix(std::fread(buf.data(), 1, buf.size(), f));
+    std::fclose(f);
+    tp.hash = h;
+    return true;
+}
+
+constexpr char CACHE_MAGIC[8] = {'F', 'B', 'C', 'C', 'S', 'R', '1', '\0'};
+
+struct CacheHeader {
+    char magic[8];
+    TextPrint text;
+    int64_t n, edges;
+    double total_weight;
+};
+
+template <typename T>
+bool read_vec(FILE *f, std::vector<T> &v, size_t count)
+{
+    v.resize(count);
+    return std::fread(v.data(), sizeof(T), count, f) == count;
+}
+
+template <typename T>
+bool write_vec(FILE *f, const std::vector<T> &v)
+{
+    return std::fwrite(v.data(), sizeof(T), v.size(), f) == v.size();
+}
+
+bool read_cache(const std::string &cache, const TextPrint &tp, Graph &g)
+{
+    FILE *f = std::fopen(cache.c_str(), "rb");
+    if (!f) return false;
+    CacheHeader h;
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, CACHE_MAGIC, 8) == 0 && h.text == tp &&
+              h.n >= 0 && h.n < INT_MAX && h.edges >= 0;
+    if (ok) {
+        g.n = (int)h.n;
+        g.total_weight = h.total_weight;
+        ok = read_vec(f, g.rp, (size_t)h.n + 1) && read_vec(f, g.col, (size_t)h.edges) &&
+             read_vec(f, g.w, (size_t)h.edges) && read_vec(f, g.in_wdeg, (size_t)h.n) &&
+             read_vec(f, g.out_wdeg, (size_t)h.n) && std::fgetc(f) == EOF;
+        ok = ok && g.rp[0] == 0 && g.rp[(size_t)h.n] == (uint64_t)h.edges;
+        for (int64_t v = 0; ok && v < h.n; ++v) ok = g.rp[v] <= g.rp[v + 1];
+        for (size_t e = 0; ok && e < g.col.size(); ++e) ok = g.col[e] >= 0 && g.col[e] < g.n;
+    }
+    std::fclose(f);
+    if (ok) finish_graph(g);
+    return ok;
+}
+
+void write_cache(const std::string &cache, const TextPrint &tp, const Graph &g)
+{
+    const std::string tmp = cache + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;   /* a cache that cannot be written is simply not kept */
+    CacheHeader h;
+    std::memcpy(h.magic, CACHE_MAGIC, 8);
+    h.text = tp;
+    h.n = g.n;
+    h.edges = g.edges;
+    h.total_weight = g.total_weight;
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && write_vec(f, g.rp) && write_vec(f, g.col) && write_vec(f, g.w) &&
+              write_vec(f, g.in_wdeg) && write_vec(f, g.out_wdeg);
+    ok = std::fclose(f) == 0 && ok;
+    if (ok)
+        std::rename(tmp.c_str(), cache.c_str());
+    else
+        std::remove(tmp.c_str());
+}
+
+} // namespace
+
+Graph load_edge_list_cached(const std::string &path, const std::string &cache_path, bool *hit)
+{
+    TextPrint tp;
+    if (!text_print(path, tp)) throw std::runtime_error("cannot open " + path);
+    Graph g;
+    if (read_cache(cache_path, tp, g)) {
+        if (hit) *hit = true;
+        return g;
+    }
+    if (hit) *hit = false;
+    g = load_edge_list_file(path);
+    write_cache(cache_path, tp, g);
+    return g;
 }
 
 Graph load_edge_list_file(const std::string &path)

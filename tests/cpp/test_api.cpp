@@ -225,6 +225,24 @@ static void load_dump(const char *path, const char *out)
     std::fclose(f);
 }
 
+/* The same dump through the binary CSR cache (fbchost::load_edge_list_cached),
+ * preceded by whether the cache was read ("hit 1") or written ("hit 0"),
+ * and the arrays the loader sums in file order. */
+static void cache_dump(const char *path, const char *cache, const char *out)
+{
+    bool hit = false;
+    fbchost::Graph g = fbchost::load_edge_list_cached(path, cache, &hit);
+    FILE *f = std::fopen(out, "w");
+    if (!f) throw std::runtime_error("cannot write " + std::string(out));
+    std::fprintf(f, "hit %d\nn %d\ntotal %.17g\n", hit ? 1 : 0, g.n, g.total_weight);
+    for (int u = 0; u < g.n; ++u) {
+        std::fprintf(f, "deg %d %.17g %.17g\n", u, g.in_wdeg[u], g.out_wdeg[u]);
+        for (uint64_t e = g.rp[u]; e < g.rp[u + 1]; ++e) std::fprintf(f, "%d %d %.17g\n", u, g.col[e], g.w[e]);
+        for (uint64_t e = g.rrp[u]; e < g.rrp[u + 1]; ++e) std::fprintf(f, "in %d %d %.17g\n", u, g.rcol[e], g.rw[e]);
+    }
+    std::fclose(f);
+}
+
 int main(int argc, char **argv)
 {
     try {
@@ -232,10 +250,15 @@ int main(int argc, char **argv)
             load_dump(argv[2], argv[3]);
             return 0;
         }
+        if (argc == 5 && std::string(argv[1]) == "cache") {
+            cache_dump(argv[2], argv[3], argv[4]);
+            return 0;
+        }
         if (argc == 5 && std::string(argv[1]) == "cpu") cpu_checks(argv[2], argv[3], argv[4]);
         else if (argc == 4 && std::string(argv[1]) == "gpu") gpu_checks(argv[2], argv[3]);
         else {
-            std::fprintf(stderr, "usage: test_api cpu <dwg9> <cfg0> <louvain0> | gpu <dwg9> <cfg0> | load <edges> <out>\n");
+            std::fprintf(stderr, "usage: test_api cpu <dwg9> <cfg0> <louvain0> | gpu <dwg9> <cfg0> | load <edges> <out> | "
+                                 "cache <edges> <cache> <out>\n");
             return 2;
         }
     } catch (const std::exception &ex) {

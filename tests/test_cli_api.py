@@ -165,3 +165,38 @@ def test_edge_list_loader_matches_oracle(tmp_path, case):
         assert got[0] == "THROW" and "greater than zero" in got[1] and "greater than zero" in exp[1]
     else:
         assert got[0] == exp[0] and got[1] == exp[1]
+
+
+def test_csr_cache(tmp_path, cfg0_files):
+    """The binary CSR cache (fbc --csr-cache): a second load reads the cache
+    and gives the same graph, file-order sums included; a changed or damaged
+    cache is not used."""
+    build_api()
+    text = open(cfg0_files[0]).read()
+    extra = "0 1 7\n4095 4094 3\n0 1 2\n"          # repeated arcs: summed in file order
+    src = tmp_path / "g.txt"
+    src.write_text(text + extra)
+    cache = str(tmp_path / "g.csr")
+
+    def dump(name):
+        out = str(tmp_path / name)
+        r = subprocess.run([API, "cache", str(src), cache, out], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+        lines = open(out).read().splitlines()
+        return lines[0], lines[1:]
+
+    h1, g1 = dump("a.txt")
+    h2, g2 = dump("b.txt")
+    assert (h1, h2) == ("hit 0", "hit 1") and g1 == g2
+    ref = str(tmp_path / "ref.txt")
+    subprocess.run([API, "load", str(src), ref], check=True)
+    fwd = [ln for ln in g1 if not ln.startswith(("deg", "in", "n ", "total"))]
+    assert fwd == open(ref).read().splitlines()[1:]
+    with open(cache, "r+b") as f:                       # damaged: cut short
+        f.truncate(os.path.getsize(cache) - 8)
+    h3, g3 = dump("c.txt")
+    assert h3 == "hit 0" and g3 == g1
+    src.write_text(text + extra + "5 6 1\n")            # a changed input
+    h4, g4 = dump("d.txt")
+    assert h4 == "hit 0" and g4 != g1
+    assert dump("e.txt") == ("hit 1", g4)
