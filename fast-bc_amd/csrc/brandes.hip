@@ -726,7 +726,10 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
 #pragma unroll
         for (int k = 0; k < RB; ++k) {
             if (!ok[k]) continue;
-            if (old[k] + su < old[k]) atomicOr(err, ERR_SIGMA);
+            if (old[k] + su < old[k])
+                atomicOr(err, ERR_SIGMA);
+            else if (old[k] + su >= (1ull << 53))
+                atomicOr(err, ERR_SIGMA_INEXACT);   /* the reference's FP64 sigma rounds here */
             if ((cnt[k] >> 32) == 1u) {
                 const uint32_t x = aadd32<SC>(lcnt, 1u);
                 S.qa[lbase + x] = w[k];

@@ -69,6 +69,9 @@ struct fbc_ctx {
      * were measured by that run (1), taken from the graph's cache (2) or
      * fixed (0) */
     uint32_t last_delta = 0, last_half_max = 0, last_tuned = 0;
+    /* the last run met a sigma of 2^53 or more (exact here; the reference's
+     * FP64 count would be rounded) */
+    uint32_t last_sigma_inexact = 0;
 };
 
 /* An autotuned launch configuration. */
@@ -394,6 +397,7 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
     ctx->last_te = 0;
     std::memset(ctx->last_counters, 0, sizeof(ctx->last_counters));
     ctx->plain_sources = ctx->plain_fallback_sources = ctx->plain_refusal = 0;
+    ctx->last_sigma_inexact = 0;
     if (p.ntasks == 0) return 0;
     p.sort_cap = ctx->sort_cap;
     uint32_t err = 0;
@@ -465,6 +469,7 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
             if (rc) return rc;
         }
     }
+    ctx->last_sigma_inexact = (err & fbcdev::ERR_SIGMA_INEXACT) ? 1 : 0;
     if (err & fbcdev::ERR_SOURCE) return fail(ctx, FBC_E_ARG, "source out of range");
     if (err & fbcdev::ERR_SIGMA)
         return fail(ctx, FBC_E_SIGMA_OVERFLOW, "shortest-path count exceeds 2^64-1");
@@ -1878,7 +1883,8 @@ int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count)
 {
     if (!ctx || !counters || count < 0) return FBC_E_ARG;
     const uint64_t host[] = {ctx->plain_sources, ctx->plain_fallback_sources, ctx->plain_refusal,
-                             ctx->last_delta,    ctx->last_half_max,           ctx->last_tuned};
+                             ctx->last_delta,    ctx->last_half_max,           ctx->last_tuned,
+                             ctx->last_sigma_inexact};
     constexpr int NH = (int)(sizeof(host) / sizeof(host[0]));
     int i = 0;
     for (; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];

@@ -36,7 +36,7 @@ HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
                  "ns_dist", "ns_sigma", "ns_dep", "sigma_wide",
                  "plain_sources", "plain_fallback_sources", "plain_refusal",
-                 "delta", "half_team_max", "tuned")
+                 "delta", "half_team_max", "tuned", "sigma_inexact")
 
 ERRORS = {
     -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",

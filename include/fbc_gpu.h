@@ -261,8 +261,11 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * refusal (0: none); then the delta and half-team limit of the last run's
  * (main) launch and how they were chosen (0: fixed or set by option, 1:
  * measured by the autotuner in this run, 2: the graph's tuned choice from an
- * earlier run).  Fills at most 'count' entries, returns how many it
- * wrote.  Instrumentation only; no counterpart in the reference. */
+ * earlier run); then 1 when the last run met a shortest-path count of 2^53
+ * or more (exact here, where the reference's FP64 count is rounded; counts
+ * of 2^64 or more return FBC_E_SIGMA_OVERFLOW).  Fills at most 'count'
+ * entries, returns how many it wrote.  Instrumentation only; no counterpart
+ * in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);
 
 #ifdef __cplusplus

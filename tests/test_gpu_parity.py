@@ -269,6 +269,8 @@ def test_unit_grid_sigma_closed_form(gpu_ctx):
     n, (s, d, w) = F.generate_grid(R, R, 1, 7, road=False)
     g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
     dist, sig = g.sssp_sigma([0])
+    # C(58, 29) ~ 3.0e16 is beyond 2^53: exact here, rounded in the reference's FP64
+    assert gpu_ctx.last_counters()["sigma_inexact"] == 1
     r, c = np.divmod(np.arange(n), R)
     assert np.array_equal(dist[0], (r + c).astype(np.uint64))
     assert all(int(sig[0][v]) == comb(int(r[v] + c[v]), int(r[v])) for v in range(n))
@@ -287,6 +289,7 @@ def test_unit_strip_sigma_beyond_48_bits(gpu_ctx):
     g, og = both_graphs(gpu_ctx, n, s, d, w)
     dist, sig = g.sssp_sigma([0])
     assert gpu_ctx.last_counters()["sigma_wide"] == 1
+    assert gpu_ctx.last_counters()["sigma_inexact"] == 0   # 1.6e15 < 2^53
     r, c = np.divmod(np.arange(n), C)
     assert np.array_equal(dist[0], (r + c).astype(np.uint64))
     assert all(int(sig[0][v]) == comb(int(r[v] + c[v]), int(r[v])) for v in range(n))
