@@ -52,11 +52,18 @@ import fastbc_amd as F  # noqa: E402
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 
 
+def big_stack():
+    """The reference keeps N-double OpenMP reduction arrays on thread stacks
+    (ClusteredBrandesBC.h:143, DijkstraClusterEvaluator.h:62): 8 MB at 1M
+    vertices, beyond the default stack limit."""
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run(args, env=None):
     e = dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="256M")
     if env:
         e.update(env)
-    r = subprocess.run(args, env=e, capture_output=True, text=True, check=True)
+    r = subprocess.run(args, env=e, capture_output=True, text=True, check=True, preexec_fn=big_stack)
     return r.stdout
 
 
@@ -156,12 +163,9 @@ def ref_dep_rows(ref, graph, sources, tmp):
             f.write(f"{int(s)} 1\n")
         op = os.path.join(tmp, f"dep{s}.txt")
         outs.append(op)
-        # the reference's array reduction over N doubles (ClusteredBrandesBC.h:143)
-        # keeps a private N-array on the thread's stack: 32 MB at 4M vertices
         procs.append(subprocess.Popen([ref, "ssb", graph, pp, op],
                                       env=dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="256M"),
-                                      preexec_fn=lambda: resource.setrlimit(
-                                          resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))))
+                                      preexec_fn=big_stack))
     for p in procs:
         assert p.wait() == 0
     return [np.loadtxt(op, comments="#") for op in outs]

@@ -278,3 +278,39 @@ def test_pivot_sum_matches_reference(cfg0):
     for p, w in zip(gold["pivots"], gold["weights"]):
         acc += cfg0.single_source_dep(int(p)) * float(w)
     assert np.allclose(acc, gold["bc"], rtol=1e-12, atol=1e-9)
+
+
+def test_cfg2_small_clusters_match_reference():
+    """The oracle at full cfg2 size (1,048,576 vertices) against the
+    reference's own runs (cfg2_reference.npz): on the three smallest
+    communities of the reference partition, borders, intra BC (bit for bit)
+    and VertexInfo rows (digests) equal DijkstraClusterEvaluator's, and the
+    oracle's selection fixture (cfg2_oracle_pivots.npz) holds the pivots that
+    the reference's k-means gives on their exact classes."""
+    import hashlib
+    ref = np.load(os.path.join(GOLDEN, "cfg2_reference.npz"))
+    sel = np.load(os.path.join(GOLDEN, "cfg2_oracle_pivots.npz"))
+    comm = ref["comm"].astype(np.int64)
+
+    def sha(a):
+        return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), np.uint8)
+
+    assert np.array_equal(sha(comm.astype(np.int32)), sel["comm_sha"])
+    n, (s, d, w) = F.generate_config(2)
+    og = O.Graph.from_arcs(s, d, w.astype(np.float64))
+    for i, c in enumerate(ref["clusters"]):
+        verts = np.where(comm == c)[0]
+        bc, borders, L, C = og.cluster_eval(verts)
+        assert np.array_equal(borders, ref[f"borders{i}"])
+        assert np.array_equal(bc[verts], ref[f"intra{i}"])
+        assert np.array_equal(sha(L.astype(np.uint32)), ref[f"len_sha{i}"])
+        assert np.array_equal(sha(C.astype(np.int32)), ref[f"cnt_sha{i}"])
+        exact, _, _ = O.select_pivots(verts, L, C, bc[verts], borders)
+        assert len(exact) == int(ref[f"classes{i}"][0])
+        expect, seen = [], set()
+        for k, wt in zip(ref[f"km_cent{i}"], ref[f"km_w{i}"]):
+            if int(exact[k]) not in seen:
+                seen.add(int(exact[k]))
+                expect.append((int(exact[k]), int(wt)))
+        mine = sel["pivot_cluster"] == c
+        assert list(zip(sel["pivots"][mine].tolist(), sel["weights"][mine].tolist())) == expect
