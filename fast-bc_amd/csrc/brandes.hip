@@ -128,6 +128,7 @@ struct Slot {
     uint64_t *key;
     uint32_t *qa, *qb, *fa, *fb;
     uint64_t *sd;
+    bool agg;   /* list appends aggregated per wavefront (team_append) */
 };
 constexpr uint32_t NEAR_TAG = 0x80000000u;
 
@@ -419,13 +420,45 @@ struct TeamGrid {
 
 template <class T> struct IsWG { static constexpr bool value = false; };
 template <> struct IsWG<TeamWG> { static constexpr bool value = true; };
+template <class T> struct IsHalf { static constexpr bool value = false; };
+template <> struct IsHalf<TeamHalf> { static constexpr bool value = true; };
+
+/* A place in a list counted by *cnt for each calling lane, one atomic per
+ * wavefront: the lanes that append -- exactly those executing this call,
+ * the others being off in the branch that leads here -- take consecutive
+ * places; the first of them adds their number and each offsets from it by
+ * the appending lanes below it.  The two half-wavefront teams of a
+ * wavefront count their own lists, so their lanes group by half. */
+template <class T>
+__device__ __forceinline__ uint32_t team_append(uint32_t *cnt)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t all = __ballot(1);   /* the calling lanes (wavefront-uniform) */
+    const uint64_t below = all & ((1ull << lane) - 1ull);
+    if constexpr (IsHalf<T>::value) {
+        constexpr uint64_t LO = 0x00000000FFFFFFFFull;
+        const uint64_t lo = all & LO, hi = all & ~LO;
+        const uint64_t mine = lane < 32 ? lo : hi;
+        uint32_t base = 0;
+        if ((below & (lane < 32 ? LO : ~LO)) == 0) base = aadd32<T::SC>(cnt, (uint32_t)__popcll(mine));
+        /* each half's first lane holds its base; both leaders are uniform */
+        const uint32_t b0 = lo ? (uint32_t)__builtin_amdgcn_readlane((int)base, __ffsll((unsigned long long)lo) - 1) : 0u;
+        const uint32_t b1 = hi ? (uint32_t)__builtin_amdgcn_readlane((int)base, __ffsll((unsigned long long)hi) - 1) : 0u;
+        return (lane < 32 ? b0 : b1) + (uint32_t)__popcll(below & (lane < 32 ? LO : ~LO));
+    } else {
+        uint32_t base = 0;
+        if (below == 0) base = aadd32<T::SC>(cnt, (uint32_t)__popcll(all));
+        /* the first calling lane is the first active one */
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + (uint32_t)__popcll(below);
+    }
+}
 
 /* Append w to a near queue whose first T::LQ entries live in LDS (lq) and
  * the rest in the slot's global queue (q). */
 template <class T>
-__device__ __forceinline__ void push_near(uint32_t *lq, uint32_t *q, uint32_t *cnt, uint32_t w)
+__device__ __forceinline__ void push_near(uint32_t *lq, uint32_t *q, uint32_t *cnt, uint32_t w, bool agg)
 {
-    const uint32_t i = aadd32<T::SC>(cnt, 1u);
+    const uint32_t i = agg ? team_append<T>(cnt) : aadd32<T::SC>(cnt, 1u);
     if (T::LQ && i < T::LQ)
         lq[i] = w;
     else
@@ -463,12 +496,12 @@ __device__ __forceinline__ void relax_arc(const View &V, const Slot S, uint32_t 
     ++ct.c[STAT_IMPROVE];
     if (nd < thr) {
         /* queued already for the next round (with a larger distance): not again */
-        if ((uint32_t)old != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, a.x);
+        if ((uint32_t)old != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, a.x, S.agg);
     } else {
         amin32<SC>(far_min, nd);
         /* on this window's far list already: not again */
         if ((uint32_t)old != win) {
-            far[aadd32<SC>(far_cnt, 1u)] = a.x;
+            far[S.agg ? team_append<T>(far_cnt) : aadd32<SC>(far_cnt, 1u)] = a.x;
             ++ct.c[STAT_FARPUSH];
         }
     }
@@ -511,11 +544,11 @@ __device__ __forceinline__ void relax_arcs(const Slot S, uint32_t du, const uint
         if ((cur[j] >> 32) <= nd) continue;
         ++ct.c[STAT_IMPROVE];
         if (nd < thr) {
-            if ((uint32_t)cur[j] != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, to[j]);
+            if ((uint32_t)cur[j] != (NEAR_TAG | nxt)) push_near<T>(lnext, qnext, qn_cnt, to[j], S.agg);
         } else {
             amin32<SC>(far_min, nd);
             if ((uint32_t)cur[j] != win) {
-                far[aadd32<SC>(far_cnt, 1u)] = to[j];
+                far[S.agg ? team_append<T>(far_cnt) : aadd32<SC>(far_cnt, 1u)] = to[j];
                 ++ct.c[STAT_FARPUSH];
             }
         }
@@ -670,12 +703,12 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             const uint32_t d = dist_of<SC>(S, w);   /* no relaxation runs during the split */
             if (d < thr) continue;            /* settled in this window */
             if (d < nthr) {
-                push_near<T>(lqa, qa, &c.qcnt[rnd % 3], w);
+                push_near<T>(lqa, qa, &c.qcnt[rnd % 3], w, S.agg);
             } else {
                 /* carried to the next window's list, tagged as on it */
                 amin32<SC>(&c.minv[nwin % 3], d);
                 ast64<SC>(&S.key[w], ((uint64_t)d << 32) | nwin);
-                fb[aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
+                fb[S.agg ? team_append<T>(&c.fcnt[nwin & 1]) : aadd32<SC>(&c.fcnt[nwin & 1], 1u)] = w;
             }
         }
         t.sync();
@@ -693,11 +726,13 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
  * the next level, at lbase + its place in that level.  Packed form: a sigma
  * carry into the count, or more arrivals than the level list holds (only
  * after such a carry or a count beyond PK_CMAX), raises *wide. */
-template <bool PACKED, int SC, int RB, uint32_t LQ>
+template <bool PACKED, class T, int RB>
 __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t (&w)[RB], const bool (&ok)[RB],
                                        uint64_t su, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
                                        uint32_t *wide, uint32_t *err)
 {
+    constexpr int SC = T::SC;
+    constexpr uint32_t LQ = T::LQ;
     if constexpr (PACKED) {
         uint64_t old[RB];
 #pragma unroll
@@ -707,7 +742,7 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
             if (!ok[k]) continue;
             if ((old[k] & PK_SIGMA) + su > PK_SIGMA) ast32<SC>(wide, 1u);
             if ((old[k] >> 48) == 1u) {
-                const uint32_t x = aadd32<SC>(lcnt, 1u), i = lbase + x;
+                const uint32_t x = S.agg ? team_append<T>(lcnt) : aadd32<SC>(lcnt, 1u), i = lbase + x;
                 if (i < n) {
                     S.qa[i] = w[k];
                     if (LQ && x < LQ) lqn[x] = w[k];
@@ -731,7 +766,7 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
             else if (old[k] + su >= (1ull << 53))
                 atomicOr(err, ERR_SIGMA_INEXACT);   /* the reference's FP64 sigma rounds here */
             if ((cnt[k] >> 32) == 1u) {
-                const uint32_t x = aadd32<SC>(lcnt, 1u);
+                const uint32_t x = S.agg ? team_append<T>(lcnt) : aadd32<SC>(lcnt, 1u);
                 S.qa[lbase + x] = w[k];
                 if (LQ && x < LQ) lqn[x] = w[k];
             }
@@ -750,7 +785,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
                                                   uint32_t mask, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
                                                   uint32_t *wide, uint32_t *err)
 {
-    constexpr int SC = T::SC, RB = T::RB;
+    constexpr int RB = T::RB;
     uint32_t m = mask;
     while (m) {
         uint32_t j[RB];
@@ -764,7 +799,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
         uint32_t w[RB];
 #pragma unroll
         for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
-        arrive<PACKED, SC, RB, T::LQ>(S, V.n, w, ok, su, lbase, lcnt, lqn, wide, err);
+        arrive<PACKED, T, RB>(S, V.n, w, ok, su, lbase, lcnt, lqn, wide, err);
     }
 }
 
@@ -788,7 +823,7 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
             tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dist_of<SC>(S, a[j].x);
             w[j] = a[j].x;
         }
-        arrive<PACKED, SC, RB, T::LQ>(S, V.n, w, tight, su, lbase, lcnt, lqn, wide, err);
+        arrive<PACKED, T, RB>(S, V.n, w, tight, su, lbase, lcnt, lqn, wide, err);
     }
 }
 
@@ -1335,6 +1370,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
     __shared__ Shared sh;
     Slot S;
     slot_layout(p.nmax, p.scratch + (uint64_t)blockIdx.x * p.slot_bytes, &S);
+    S.agg = p.ballot_append != 0;
     const TeamWG t{&sh.ctl, &sh.loc, sh.sortbuf, sh.lq};
     if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;
@@ -1361,6 +1397,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (blockIdx.x * WAVES + w >= p.nslots) return;   /* no slot behind this wavefront */
     Slot S;
     slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * WAVES + w) * p.slot_bytes, &S);
+    S.agg = p.ballot_append != 0;
     const TeamWave t{&sw.ctl[w], &sw.loc[w], sw.lq[w]};
     if (t.lleader()) sw.loc[w].te = 0;
     Counters ct;
@@ -1384,6 +1421,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (blockIdx.x * HALVES + h >= p.nslots) return;   /* no slot behind this half */
     Slot S;
     slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * HALVES + h) * p.slot_bytes, &S);
+    S.agg = p.ballot_append != 0;
     const TeamHalf t{&sh.ctl[h], &sh.loc[h], sh.lq[h]};
     if (t.lleader()) sh.loc[h].te = 0;
     Counters ct;
@@ -1405,6 +1443,7 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     __shared__ Shared sh;
     Slot S;
     slot_layout(p.nmax, p.scratch, &S);
+    S.agg = p.ballot_append != 0;
     const TeamGrid t{reinterpret_cast<Ctl *>(p.grid_ctl), &sh.loc, nullptr};
     if (threadIdx.x == 0) sh.loc.te = 0;
     Counters ct;

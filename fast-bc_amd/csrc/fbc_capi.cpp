@@ -65,6 +65,7 @@ struct fbc_ctx {
     double wall_khz = 100000.0;   /* device wall clock (wall_clock64) rate */
     uint8_t *noalloc = nullptr;   /* this device's address of 'scratch' */
     uint32_t autotune = 1;        /* tune delta / team form on large batched runs */
+    uint32_t ballot_append = 0;   /* wavefront-aggregated list appends */
     /* the last batched run: its delta and half-team limit, and whether they
      * were measured by that run (1), taken from the graph's cache (2) or
      * fixed (0) */
@@ -400,6 +401,7 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
     ctx->last_sigma_inexact = 0;
     if (p.ntasks == 0) return 0;
     p.sort_cap = ctx->sort_cap;
+    p.ballot_append = ctx->ballot_append;
     uint32_t err = 0;
     int rc = 0;
     if (!plain) {
@@ -564,6 +566,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
     else if (!std::strcmp(name, "half_team_max")) ctx->half_max = std::min<uint32_t>(v, fbcdev::WAVE_TEAM_MAX_N);
     else if (!std::strcmp(name, "autotune")) ctx->autotune = v ? 1 : 0;
+    else if (!std::strcmp(name, "ballot_append")) ctx->ballot_append = v ? 1 : 0;
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
@@ -921,6 +924,7 @@ int fbc_dep_batch_noalloc(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *d_sr
     p.half_max = ctx->half_max;
     p.delta = auto_delta(ctx, g);
     p.sort_cap = ctx->sort_cap;
+    p.ballot_append = ctx->ballot_append;
     p.queue_head = reinterpret_cast<uint32_t *>(base);
     p.err = reinterpret_cast<uint32_t *>(base + 4);
     p.te = reinterpret_cast<unsigned long long *>(base + 8);

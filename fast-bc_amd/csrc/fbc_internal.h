@@ -86,6 +86,7 @@ struct BrandesParams {
     uint32_t sort_cap;        /* batched kernel: sort near queues / levels of up to this many vertices (0: off) */
     uint32_t nslots;          /* slots the scratch holds (the wavefront kernels check their slot against it) */
     uint32_t half_max;        /* views up to this many vertices: half-wavefront teams */
+    uint32_t ballot_append;   /* 1: queue and level appends take one atomic per wavefront */
 };
 
 /* Largest near queue or level the batched kernel sorts (LDS words). */

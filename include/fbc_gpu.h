@@ -88,7 +88,11 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * enough for half-wavefront teams also whole wavefronts -- keeps the
  * candidate with the most traversed edges per millisecond for the rest of
  * the run and for later runs on the graph; an explicit "delta" turns it
- * off; 0: off).  No counterpart in the reference. */
+ * off; 0: off), "ballot_append" (1: queue and level appends take one
+ * atomic per wavefront -- a ballot of the appending lanes, the first adds
+ * their count -- instead of one per lane; same results; 0, the default,
+ * because it measured 0.5-1% slower on the 1M-vertex grid).  No counterpart
+ * in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].

@@ -339,3 +339,31 @@ def test_cfg2_dist_sigma_properties(gpu_ctx):
         np.add.at(acc, d64[tight], sig[i][s64[tight]])
         acc[src] = 1
         assert np.array_equal(acc, sig[i])
+
+
+def test_ballot_append_same_results(gpu_ctx):
+    """Wavefront-aggregated appends (one atomic per wavefront for the near
+    queues, far lists and DAG levels) change only the order of the lists:
+    distances, sigma and the ordered FP64 dependencies stay bit-exact, for
+    wavefront teams (cfg0), half-wavefront teams (cfg0's cluster views),
+    workgroup teams (a 1M-vertex grid) and the plain form."""
+    n, (s, d, w) = F.generate_config(0)
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    comm, _ = F.louvain(s, d, w, [1, 2, 3, 4], 0.01)
+    cl = g.clusters(comm_of=comm)
+    srcs = np.random.default_rng(9).choice(n, 24, replace=False)
+    n2, arcs2 = F.generate_config(2)
+    g2 = F.Graph(gpu_ctx, n2, *F.csr_from_arcs(n2, *arcs2))
+    big = np.array([0, 523777], np.uint32)
+    base = (g.dep_batch(srcs), cl.evaluate(), g2.dep_batch(big))
+    gpu_ctx.set_option("ballot_append", 1)
+    try:
+        agg = (g.dep_batch(srcs), cl.evaluate(), g2.dep_batch(big))
+        plain = g.single_source_dep(int(srcs[0]))
+        assert_plain_ran(gpu_ctx, 1)
+        check_sssp(g, og, srcs[:6])
+    finally:
+        gpu_ctx.set_option("ballot_append", 0)
+    assert np.array_equal(agg[0], base[0]) and np.array_equal(agg[2], base[2])
+    assert all(np.array_equal(a, b) for a, b in zip(agg[1], base[1]))
+    assert np.array_equal(plain, og.single_source_dep(int(srcs[0])))
