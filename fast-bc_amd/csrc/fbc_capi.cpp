@@ -194,16 +194,17 @@ uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
 int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out, uint32_t half_max)
 {
     const uint64_t sb = fbcdev::slot_bytes(nmax);
-    /* whole-graph slots: 4 per CU (throughput levels off near 3); a
-     * wavefront-per-source workgroup holds 4 slots, 8 such workgroups per CU */
+    /* whole-graph slots: 8 per CU (on the 1M-vertex grid's pivot phase 4, 6, 8
+     * and 12 per CU took 3277, 3206, 3159 and 3162 ms); a wavefront-per-source
+     * workgroup holds 4 slots, 8 such workgroups per CU */
     const int kind = fbcdev::team_kind(nmax, half_max);
-    const uint32_t per_cu = kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 4u;
+    const uint32_t per_cu = kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 8u;
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
     /* slots (40 B per vertex each) are also bounded by memory: half the
      * device, less what is not free now (the current scratch aside) and a
-     * sixteenth kept for the caller (4 per CU of a 1M-vertex graph take 41 GB;
+     * sixteenth kept for the caller (8 per CU of a 1M-vertex graph take 86 GB;
      * a 4M-vertex graph gets ~3 per CU, whose pivot phase runs 12% faster than
      * with a third of the device) */
     size_t budget = ctx->scratch_mb ? (size_t)ctx->scratch_mb << 20 : ctx->total_mem / 2;
@@ -313,7 +314,7 @@ int tune_slot(int mode)
  * with no tuned configuration yet tune one on their own tasks: the first
  * K * T tasks form K interleaved trial sets (task k, k + K, k + 2K, ...), one
  * per candidate (delta of 4, 2 and 8 mean arc weights; for views small
- * enough for half-wavefront teams, also whole wavefronts), T = two tasks per
+ * enough for half-wavefront teams, also whole wavefronts), T = one task per
  * slot, each set one full launch; the candidate with the most traversed
  * edges per millisecond runs the remaining tasks and is kept for the graph.
  * No task is run twice. */
@@ -347,7 +348,7 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
         uint32_t slots = 0;
         int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max);
         if (rc) return rc;
-        const uint64_t T = 2ull * slots;
+        const uint64_t T = slots;
         if ((uint64_t)p.ntasks >= 2 * K * T) {
             double best = -1;
             for (uint32_t k = 0; k < K; ++k) {

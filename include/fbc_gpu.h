@@ -75,7 +75,7 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * (1, the default: the batched pivot phase and fbc_exact_bc run on a copy of
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
  * locality; 0: on the graph as given), "scratch_mb" (device memory for the
- * sources in flight, in MiB; automatic: up to 4 whole-graph sources per CU
+ * sources in flight, in MiB; automatic: up to 8 whole-graph sources per CU
  * within half the device, less what is not free and a sixteenth kept back),
  * "class_key_bits" (bits of the row hashes fbc_cluster_classes
  * keeps; 0: all 64; a few bits force collisions, a test hook),
