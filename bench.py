@@ -17,7 +17,7 @@ Beside the headline, in the same JSON line:
                    the largest relative BC difference between the two forms;
   plain_fp64_form  the pivot phase as the reference runs it, one
                    singleSourceBrandes per pivot -- here one cooperative
-                   whole-GPU launch per pivot, FP64 -- timed over a sample of
+                   launch of 16 workgroups per pivot, FP64 -- timed over a sample of
                    the run's pivots and scaled to all of them; the other
                    phases are the batched form's (a conservative baseline);
   roofline         the pivot-phase launch (the dominant kernel): 48
@@ -97,6 +97,15 @@ def cpu_phases(ref, gpath, n, arcs, comm, cores, tmp):
                               f"n_c*m_c over {cores} cores"}
 
 
+def cpu_model():
+    """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
+    try:
+        names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(n, arcs, sources, seconds_target, comm):
     """Time the reference's single-source Brandes on the host (bounded sample)."""
     import numpy as np
@@ -156,10 +165,138 @@ def cpu_baseline(n, arcs, sources, seconds_target, comm):
         cores = 1
         kind = "port"
         sample = f"{k} of the run's pivots through the oracle's single-source Brandes, one thread, {secs:.1f} s"
-    out = {"value": k * m / secs / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind, "sample": sample}
+    out = {"value": k * m / secs / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind, "sample": sample,
+           "cpu_model": cpu_model()}
     if kind == "reference":
         out["end_to_end"] = est
     return out
+
+
+def ref_exact(n, arcs, cores, sample_sources=None):
+    """The reference's ExactBrandesBC on the host: every source when
+    sample_sources is None (returns its seconds and BC), else
+    DijkstraSSBrandesBC over that many sources spread over the vertices, the
+    seconds scaled to all n sources (returns the estimate and no BC)."""
+    import numpy as np
+    import fastbc_amd as F
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver_fast")
+    if not os.path.exists(ref):
+        return None, None
+    tmp = tempfile.mkdtemp(prefix="fbc_table_")
+    gpath = os.path.join(tmp, "graph.txt")
+    out = os.path.join(tmp, "out.txt")
+    F.write_edge_list(n, arcs, gpath)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_STACKSIZE="256M")
+    if sample_sources is None:
+        r = subprocess.run([ref, "exact", gpath, out], env=env, capture_output=True, text=True, timeout=900,
+                           preexec_fn=big_stack)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        secs = float(r.stderr.split("seconds=")[1].split()[0])
+        return secs, np.loadtxt(out)
+    ppath = os.path.join(tmp, "piv.txt")
+    with open(ppath, "w") as f:
+        for v in range(0, n, max(1, n // sample_sources))[:sample_sources]:
+            f.write(f"{v} 1\n")
+    r = subprocess.run([ref, "ssb", gpath, ppath, out], env=env, capture_output=True, text=True, timeout=900,
+                       preexec_fn=big_stack)
+    if r.returncode:
+        raise RuntimeError(r.stderr)
+    return float(open(out).readline().split()[2]) * n / sample_sources, None
+
+
+def table(args):
+    """One JSON line per (configuration, form): computeBC wall time (steady
+    state: each form's first call on a graph is a warm-up) and GTEPS of the
+    batched FP32C and FP64 forms, FP32C's largest relative difference from
+    FP64, the plain FP64 form (every pivot when there are at most 3,000, else
+    a strided sample of 32 scaled to all), and the reference's own code on
+    the host cores (every source of cfg0's exact BC; a sample for the rest)."""
+    import numpy as np
+    import fastbc_amd as F
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 16))
+
+    def emit(**kw):
+        print(json.dumps(kw), flush=True)
+
+    def rel(a, b):
+        nz = b != 0
+        return float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz]))) if nz.any() else 0.0
+
+    for cfg, mode, kfrac in ((0, "exact", 0), (0, "clustered", 0.5), (1, "clustered", 0.5), (3, "exact", 0),
+                             (2, "clustered", 0.5), (4, "clustered", 0.25)):
+        n, arcs = F.generate_config(cfg)
+        m = len(arcs[0])
+        log(f"cfg{cfg} {mode}: n={n} m={m}")
+        if mode == "exact":
+            res = {}
+            for name, prec in (("fp64", F.PREC_F64), ("fp32c", F.PREC_F32C)):
+                F.exact_bc_arcs(*arcs, prec=prec)
+                t = time.perf_counter()
+                res[name] = F.exact_bc_arcs(*arcs, prec=prec)
+                secs = time.perf_counter() - t
+                emit(config=f"cfg{cfg}", mode=mode, form=f"GPU batched {name}", wall_s=round(secs, 4),
+                     gteps=round(n * m / secs / 1e9, 3), traversed_edges=n * m,
+                     fp32c_max_rel_diff_vs_fp64=rel(res["fp32c"], res["fp64"]) if name == "fp32c" else None)
+            F.host_release()
+            secs, ref_bc = ref_exact(n, arcs, cores, None if cfg == 0 else 16 * cores)
+            if secs is not None:
+                emit(config=f"cfg{cfg}", mode=mode, form=f"reference CPU ({cores} cores)", wall_s=round(secs, 2),
+                     gteps=round(n * m / secs / 1e9, 4), cores=cores, cpu_model=cpu_model(),
+                     sample="every source" if ref_bc is not None else f"{16 * cores} sources, scaled to {n}",
+                     gpu_fp64_max_rel_err_vs_reference=rel(res["fp64"], ref_bc) if ref_bc is not None else None)
+            continue
+        runs = {}
+        for name, prec in (("fp32c", F.PREC_F32C), ("fp64", F.PREC_F64)):
+            if name == "fp32c":
+                t = time.perf_counter()
+                F.clustered_bc(*arcs, SEEDS, 0.01, kfrac=kfrac, prec=prec)
+                log(f"  cold first call {time.perf_counter() - t:.1f} s")
+            r = F.clustered_bc(*arcs, SEEDS, 0.01, kfrac=kfrac, prec=prec)
+            runs[name] = r
+            ph = r["phases"]
+            te = ph["cluster_te"] + ph["pivot_te"]
+            emit(config=f"cfg{cfg}", mode=f"{mode} -k {kfrac}", form=f"GPU batched {name}",
+                 wall_s=round(ph["total_s"], 3), gteps=round(te / ph["total_s"] / 1e9, 3),
+                 pivot_kernel_gteps=round(ph["pivot_te"] / (ph["pivot_kernel_ms"] * 1e-3) / 1e9, 3),
+                 roofline_frac=round(ph["pivot_te"] * BYTES_PER_TE / (ph["pivot_kernel_ms"] * 1e-3) / 1e9
+                                     / HBM_PEAK_GBS, 4),
+                 phases_s={k: round(v, 3) for k, v in ph.items() if k.endswith("_s")},
+                 clusters=int(np.max(r["comm"])) + 1, pivots=len(r["pivots"]), traversed_edges=te,
+                 fp32c_max_rel_diff_vs_fp64=rel(runs["fp32c"]["bc"], r["bc"]) if name == "fp64" else None)
+        r64 = runs["fp64"]
+        p64 = r64["phases"]
+        F.host_release()
+        ctx = F.Context(0)
+        g = F.Graph(ctx, n, *F.csr_from_arcs(n, *arcs))
+        cl = g.clusters(comm_of=r64["comm"])
+        npiv = len(r64["pivots"])
+        k = npiv if npiv <= 3000 else 32
+        stride = max(1, npiv // k)
+        sp, sw, sc = (a[::stride][:k] for a in (r64["pivots"], r64["weights"], r64["pivot_cluster"]))
+        zero = np.zeros(n)
+        plain = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, plain=True)
+        plain_ms, _ = ctx.last_stats()
+        pc = ctx.last_counters()
+        batched = cl.pivot_phase(sp, sw, sc, r64["intra"], zero, prec=F.PREC_F64)
+        agree = float(np.max(np.abs(plain - batched)) / max(np.abs(batched).max(), 1e-300))
+        plain_pivot_s = plain_ms * 1e-3 / k * npiv
+        plain_total = p64["total_s"] - p64["pivot_phase_s"] + plain_pivot_s
+        emit(config=f"cfg{cfg}", mode=f"{mode} -k {kfrac}", form="GPU plain fp64 (one cooperative launch per pivot)",
+             wall_s=round(plain_total, 2), gteps=round((p64["cluster_te"] + p64["pivot_te"]) / plain_total / 1e9, 4),
+             ms_per_pivot=round(plain_ms / k, 3),
+             sample="every pivot" if k == npiv else f"{k} of {npiv} pivots (every {stride}th), scaled",
+             cooperative_launches=pc["plain_sources"], batched_fallback_sources=pc["plain_fallback_sources"],
+             max_diff_vs_batched_fp64=agree)
+        del cl, g, ctx
+        if cfg != 2:   # the default bench line carries cfg2's
+            try:
+                cb = cpu_baseline(n, arcs, r64["pivots"], 10.0, r64["comm"])
+                emit(config=f"cfg{cfg}", mode=f"{mode} -k {kfrac}", form=f"reference CPU ({cb['cores']} cores)",
+                     **{k2: v for k2, v in cb.items() if k2 != "unit"})
+            except Exception as e:
+                log(f"cpu baseline failed: {e}")
 
 
 def traffic_per_launch(npiv):
@@ -184,6 +321,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 and plain forms")
+    ap.add_argument("--table", action="store_true",
+                    help="measure every benchmark configuration (one JSON line per configuration and form)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -200,6 +339,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build measures a single "
               f"MI355X; the pivot and (cluster, source) queues would be split across GPUs", file=sys.stderr)
         sys.exit(2)
+
+    if args.table:
+        table(args)
+        return
 
     import numpy as np
     import fastbc_amd as F

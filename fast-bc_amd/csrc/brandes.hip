@@ -1295,10 +1295,11 @@ __device__ __forceinline__ void flush_counters(const BrandesParams &p, const T &
             if (ct.c[i]) atomicAdd(&p.stats[i], ct.c[i]);
 }
 
-/* One (view, source) task of the batched kernels, run by team t on slot S. */
+/* One (view, source) task of the batched kernels, run by team t on slot S;
+ * q is its queue index (which trial set it belongs to in a trial launch). */
 template <int MODE, int PREC, bool ORDERED, class T>
 __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, const T &t, uint32_t task,
-                                         Counters &ct)
+                                         uint32_t q, Counters &ct)
 {
     View V;
     uint32_t c = 0;
@@ -1324,8 +1325,11 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
         return;
     }
 
+    const uint32_t set = p.trial_sets ? q % p.trial_sets : 0u;
+    const uint32_t delta = p.trial_sets ? p.trial_delta[set] : p.delta;
+    const unsigned long long te0 = t.loc->te;   /* stable: the last task ended with barriers */
     const uint64_t t0 = wall_clock64();
-    phase_distances(V, S, t, src, p.delta, p.sort_cap, p.err, ct);
+    phase_distances(V, S, t, src, delta, p.sort_cap, p.err, ct);
     const uint64_t t1 = wall_clock64();
     const uint32_t nlev = phase_sigma(V, S, t, src, p.sort_cap, p.err, ct);
     const uint64_t t2 = wall_clock64();
@@ -1358,7 +1362,14 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
         }
     }
     phase_dependencies<MODE, PREC, ORDERED>(p, V, S, t, task, c, src, nlev);
-    if (t.leader()) ct.c[STAT_T_DEP] += wall_clock64() - t2;
+    if (t.leader()) {
+        const uint64_t t3 = wall_clock64();
+        ct.c[STAT_T_DEP] += t3 - t2;
+        if (p.trial_sets) {   /* te is final: the sigma sweep's adds ended before its barrier */
+            atomicAdd(&p.trial_stats[2 * set], t.loc->te - te0);
+            atomicAdd(&p.trial_stats[2 * set + 1], (unsigned long long)(t3 - t0));
+        }
+    }
 }
 
 /* ---- the batched kernels: a device work queue of (view, source) tasks ------ */
@@ -1380,7 +1391,7 @@ __global__ __launch_bounds__(WG) void k_brandes(BrandesParams p)
         const uint32_t task = sh.task;
         __syncthreads();
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task, task, ct);
     }
     flush_counters(p, t, ct);
 }
@@ -1406,7 +1417,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
         task = __shfl(task, 0, 64);
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task, task, ct);
     }
     flush_counters(p, t, ct);
 }
@@ -1430,7 +1441,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (t.lleader()) task = atomicAdd(p.queue_head, 1u);
         task = __shfl(task, (int)(threadIdx.x & 32), 64);
         if (task >= p.ntasks) break;
-        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task * p.task_stride, ct);
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task, task, ct);
     }
     flush_counters(p, t, ct);
 }

@@ -59,6 +59,7 @@ struct fbc_ctx {
     size_t scratch_bytes = 0;
     uint32_t *d_queue = nullptr, *d_err = nullptr;
     unsigned long long *d_te = nullptr, *d_stats = nullptr;
+    unsigned long long *d_trial = nullptr;   /* autotuner: edges and busy ticks per trial set */
     uint32_t *d_grid_ctl = nullptr;   /* plain kernel: grid-shared traversal counters */
     double last_ms = 0, last_te = 0;
     unsigned long long last_counters[fbcdev::STAT_COUNT] = {};
@@ -250,15 +251,17 @@ void base_params(BrandesParams &p, const fbc_graph *g)
     p.rrp = g->d_rrp;
     p.n = g->n;
     p.nmax = g->n;
-    p.task_stride = 1;
 }
 
 /* Statuses with which the runtime refuses a cooperative launch for want of
- * device resources rather than for a bad launch. */
+ * device resources rather than for a bad launch.  hipErrorIllegalState is
+ * what a process gets for its first cooperative launch while another process
+ * on the device has made cooperative launches of its own (whatever the grid
+ * size: 16 and 256 workgroups alike). */
 bool refused_cooperative(hipError_t e)
 {
     return e == hipErrorCooperativeLaunchTooLarge || e == hipErrorNotSupported ||
-           e == hipErrorLaunchOutOfResources || e == hipErrorOutOfMemory;
+           e == hipErrorLaunchOutOfResources || e == hipErrorOutOfMemory || e == hipErrorIllegalState;
 }
 
 /* One launch of the batched kernel over q's tasks with q's delta and half-team
@@ -311,13 +314,26 @@ int tune_slot(int mode)
 
 /* The batched form of a run.  Large runs of the additive modes (pivot phase,
  * exact BC, cluster evaluation: every task's output is its own) on a graph
- * with no tuned configuration yet tune one on their own tasks: the first
- * K * T tasks form K interleaved trial sets (task k, k + K, k + 2K, ...), one
- * per candidate (delta of 4, 2 and 8 mean arc weights; for views small
- * enough for half-wavefront teams, also whole wavefronts), T = one task per
- * slot, each set one full launch; the candidate with the most traversed
- * edges per millisecond runs the remaining tasks and is kept for the graph.
- * No task is run twice. */
+ * with no tuned configuration yet tune one on their own first tasks, then
+ * keep it for the graph.  No task runs twice.
+ *
+ * 1. Region: one launch of one task per slot in which task q runs with delta
+ *    candidate q mod C (powers of two of the mean arc weight, 8 down to 1/16,
+ *    distinct as integers); the kernel adds each task's traversed edges and
+ *    busy time to its candidate, and the most edges per busy tick wins.
+ *    Candidates apart by large factors separate reliably this way (a
+ *    32K-vertex R-MAT graph whose weights average 128: exact BC 1916 / 1218 /
+ *    1197 / 1357 ms at delta 511 / 8 / 12 / 32), but a greedier candidate
+ *    takes memory bandwidth from the others in a mixed launch, so close ones
+ *    are ranked by:
+ * 2. Walk: launches of one task per slot, all at one delta, by edges per
+ *    millisecond: the region's choice, then its smaller neighbours while
+ *    they win, else its larger ones while they win (the 1M-vertex grid's
+ *    pivot phase: 3,223 ms at 2 mean weights, 3,312 at 4).
+ * 3. Team form, for views small enough for half-wavefront teams: one launch
+ *    of whole wavefronts at the chosen delta against the walk's figure.
+ * Each step runs only while a full wave of tasks would remain for the final
+ * launch; runs of fewer than two tasks per slot are not tuned. */
 int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, BrandesParams &p, uint32_t *err)
 {
     TuneChoice def;
@@ -331,53 +347,97 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
         use = g->tuned[ts];
         ctx->last_tuned = 2;
     } else if (tunable) {
-        std::vector<TuneChoice> cand;
-        for (double m : {4.0, 2.0, 8.0}) {
-            TuneChoice c;
-            c.delta = (uint32_t)std::min(std::max(g->meanw * m, 1.0), 1e9);
-            c.half_max = def.half_max;
-            cand.push_back(c);
-        }
-        if (fbcdev::team_kind(p.nmax, def.half_max) == fbcdev::TEAM_HALF) {
-            TuneChoice c;
-            c.delta = cand[0].delta;
-            c.half_max = 0;
-            cand.push_back(c);
-        }
-        const uint32_t K = (uint32_t)cand.size();
         uint32_t slots = 0;
         int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max);
         if (rc) return rc;
-        const uint64_t T = slots;
-        if ((uint64_t)p.ntasks >= 2 * K * T) {
-            double best = -1;
-            for (uint32_t k = 0; k < K; ++k) {
+        if ((uint64_t)p.ntasks >= 2ull * slots) {
+            std::vector<uint32_t> cand;   /* descending */
+            for (double m = 8.0; m >= 1.0 / 16 && cand.size() < fbcdev::TRIAL_MAX; m *= 0.5) {
+                const uint32_t d = (uint32_t)std::min(std::max(g->meanw * m, 1.0), 1e9);
+                if (cand.empty() || d < cand.back()) cand.push_back(d);
+            }
+            const uint32_t C = (uint32_t)cand.size();
+            uint64_t done = 0;
+            /* 1. region */
+            {
                 BrandesParams q = p;
-                q.task_base = k;
-                q.task_stride = K;
-                q.ntasks = (uint32_t)T;
-                q.delta = cand[k].delta;
-                q.half_max = cand[k].half_max;
+                q.ntasks = slots;
+                q.half_max = def.half_max;
+                q.trial_sets = C;
+                for (uint32_t k = 0; k < C; ++k) q.trial_delta[k] = cand[k];
+                q.trial_stats = ctx->d_trial;
+                HIP_TRY(ctx, hipMemsetAsync(ctx->d_trial, 0, 2 * fbcdev::TRIAL_MAX * sizeof(unsigned long long),
+                                            ctx->stream));
+                if ((rc = timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr))) return rc;
+                done += slots;
+            }
+            unsigned long long st[2 * fbcdev::TRIAL_MAX] = {};
+            HIP_TRY(ctx, hipMemcpy(st, ctx->d_trial, sizeof(st), hipMemcpyDeviceToHost));
+            uint32_t bi = 0;
+            double br = -1;
+            for (uint32_t k = 0; k < C; ++k) {
+                const double rate = st[2 * k + 1] ? (double)st[2 * k] / (double)st[2 * k + 1] : 0.0;
+                if (rate > br) {
+                    br = rate;
+                    bi = k;
+                }
+            }
+            /* one launch of n tasks at one configuration; its edges per ms */
+            auto pure = [&](uint32_t delta, uint32_t half_max, uint32_t n, double *rate) -> int {
+                BrandesParams r = p;
+                r.task_base = (uint32_t)done;
+                r.ntasks = n;
+                r.delta = delta;
+                r.half_max = half_max;
                 double ms = 0, te = 0;
-                rc = timed_launch(ctx, mode, prec, ordered, q, err, &ms, &te);
-                if (rc) return rc;
-                const double rate = te / std::max(ms, 1e-3);
-                if (rate > best) {
-                    best = rate;
-                    use = cand[k];
+                const int e = timed_launch(ctx, mode, prec, ordered, r, err, &ms, &te);
+                if (e) return e;
+                done += n;
+                *rate = te / std::max(ms, 1e-3);
+                return 0;
+            };
+            auto room = [&](uint64_t n) { return p.ntasks - done >= n + slots; };
+            /* 2. walk */
+            use.delta = cand[bi];
+            double best = -1;
+            if (room(slots)) {
+                if ((rc = pure(cand[bi], def.half_max, slots, &best))) return rc;
+                uint32_t at = bi;
+                for (int dir : {+1, -1}) {   /* +1: smaller deltas */
+                    bool moved = false;
+                    for (int k = (int)at + dir; k >= 0 && k < (int)C && room(slots); k += dir) {
+                        double r = 0;
+                        if ((rc = pure(cand[(uint32_t)k], def.half_max, slots, &r))) return rc;
+                        if (r <= best) break;
+                        best = r;
+                        at = (uint32_t)k;
+                        moved = true;
+                    }
+                    if (moved) break;
+                }
+                use.delta = cand[at];
+            }
+            /* 3. team form */
+            if (best > 0 && fbcdev::team_kind(p.nmax, def.half_max) == fbcdev::TEAM_HALF) {
+                uint32_t wave_slots = 0;
+                if ((rc = ensure_scratch(ctx, p.nmax, p.ntasks, &wave_slots, 0))) return rc;
+                if (room(wave_slots)) {
+                    double r = 0;
+                    if ((rc = pure(use.delta, 0, wave_slots, &r))) return rc;
+                    if (r > best) use.half_max = 0;
                 }
             }
             use.set = true;
             g->tuned[ts] = use;
             ctx->last_tuned = 1;
-            BrandesParams q = p;
-            q.task_base = (uint32_t)(K * T);
-            q.ntasks = p.ntasks - (uint32_t)(K * T);
-            q.delta = use.delta;
-            q.half_max = use.half_max;
             ctx->last_delta = use.delta;
             ctx->last_half_max = use.half_max;
-            return q.ntasks ? timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr) : 0;
+            BrandesParams r = p;
+            r.task_base = (uint32_t)done;
+            r.ntasks = p.ntasks - (uint32_t)done;
+            r.delta = use.delta;
+            r.half_max = use.half_max;
+            return r.ntasks ? timed_launch(ctx, mode, prec, ordered, r, err, nullptr, nullptr) : 0;
         }
     }
     ctx->last_delta = use.delta;
@@ -430,7 +490,11 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         uint32_t blocks = 0;
         HIP_TRY(ctx, fbcdev::plain_max_blocks(mode, ctx->cu_count, &blocks));
-        const uint32_t want = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count;
+        /* 16 workgroups unless set: every near round and DAG level ends in a
+         * grid barrier, whose cost grows with the grid while a thin frontier
+         * leaves most workgroups idle (on the 1M-vertex grid's pivots 82 ms per
+         * source at 16 workgroups, 228 ms at one per CU) */
+        const uint32_t want = ctx->slots_override ? ctx->slots_override : std::min<uint32_t>(16u, ctx->cu_count);
         if (blocks > want) blocks = want;
         p.grid_ctl = ctx->d_grid_ctl;
         uint32_t t = 0;
@@ -444,7 +508,10 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
              * the batched kernel; any other failure is returned. */
             if (t != 0 || !refused_cooperative(e)) {
                 (void)hipStreamSynchronize(ctx->stream);
-                return hip_fail(ctx, e, "cooperative launch of the plain form");
+                const std::string what = "cooperative launch of the plain form (source " + std::to_string(t) +
+                                         " of " + std::to_string(p.ntasks) + ", " + std::to_string(blocks) +
+                                         " workgroups, HIP status " + std::to_string((int)e) + ")";
+                return hip_fail(ctx, e, what.c_str());
             }
             ctx->plain_refusal = (uint64_t)e;
             break;
@@ -507,6 +574,7 @@ int fbc_create(int device, fbc_ctx **out)
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         dalloc(&ctx->d_queue, 1) != hipSuccess || dalloc(&ctx->d_err, 1) != hipSuccess ||
         dalloc(&ctx->d_te, 1) != hipSuccess || dalloc(&ctx->d_stats, fbcdev::STAT_COUNT) != hipSuccess ||
+        dalloc(&ctx->d_trial, 2 * fbcdev::TRIAL_MAX) != hipSuccess ||
         dalloc(&ctx->d_grid_ctl, 16) != hipSuccess || hipMemset(ctx->d_grid_ctl, 0, 16 * sizeof(uint32_t)) != hipSuccess) {
         fbc_destroy(ctx);
         return FBC_E_HIP;
@@ -538,6 +606,7 @@ void fbc_destroy(fbc_ctx *ctx)
     dfree(ctx->d_err);
     dfree(ctx->d_te);
     dfree(ctx->d_stats);
+    dfree(ctx->d_trial);
     dfree(ctx->d_grid_ctl);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -36,6 +36,9 @@ enum ErrBits : uint32_t {
     ERR_SOURCE = 8u         /* a task's source id is not a vertex of its view */
 };
 
+/* delta candidates of one autotuner trial launch at most */
+constexpr uint32_t TRIAL_MAX = 8;
+
 /* Everything a launch needs, passed by value. */
 struct BrandesParams {
     /* graph view used when cl_n == nullptr */
@@ -57,10 +60,16 @@ struct BrandesParams {
     const uint64_t *cl_doff;  /* dependency-matrix offset per cluster (this chunk) */
 
     /* tasks: source (local id in cluster mode), aux = cluster index or weight;
-     * the batched kernels' queue index q runs task task_base + q * task_stride
-     * (the autotuner's interleaved trial sets; 0 and 1 otherwise) */
+     * the batched kernels' queue index q runs task task_base + q (the
+     * autotuner's launches each take the next run of tasks) */
     uint32_t ntasks;
-    uint32_t task_base, task_stride;
+    uint32_t task_base;
+    /* the autotuner's trial launch (trial_sets > 0): queue index q runs with
+     * delta trial_delta[q % trial_sets] and adds its traversed edges and busy
+     * wall-clock ticks to trial_stats[2 (q % trial_sets)] and [... + 1] */
+    uint32_t trial_sets;
+    uint32_t trial_delta[TRIAL_MAX];
+    unsigned long long *trial_stats;
     const uint32_t *task_src;
     const uint32_t *task_aux;
     uint32_t *queue_head;

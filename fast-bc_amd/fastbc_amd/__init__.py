@@ -375,7 +375,7 @@ class Clusters:
         return flat[self.vi_off[c]:self.vi_off[c] + nc * nb].reshape(nc, nb)
 
     def pivot_phase(self, pivots, weights, pivot_cluster, intra_bc, bc, prec=PREC_F64, plain=False):
-        """bc + the pivot loop's terms.  plain=True: one whole-GPU launch per
+        """bc + the pivot loop's terms.  plain=True: one cooperative launch per
         pivot, FP64 (the baseline form); otherwise all pivots in one launch."""
         out = np.ascontiguousarray(bc, np.float64).copy()
         p = _u32(pivots)
@@ -448,7 +448,7 @@ FORM_BATCHED, FORM_PLAIN, FORM_SELECT_ONLY = 0, 1, 2
 def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, device=0,
                  form=FORM_BATCHED, partition=None):
     """End-to-end ClusteredBrandeBC::computeBC on the GPU; returns a dict.
-    form: FORM_BATCHED (all pivots in one launch), FORM_PLAIN (one whole-GPU
+    form: FORM_BATCHED (all pivots in one launch), FORM_PLAIN (one cooperative
     launch per pivot, FP64) or FORM_SELECT_ONLY (stop after pivot selection;
     bc is then the intra-cluster BC).  partition: a community index per
     vertex in place of Louvain (negative: in no community; seeds and

@@ -14,7 +14,7 @@
  *         --device             HIP device (default 0)
  *         --precision          f64 | f32c: dependency arithmetic (default f64)
  *         --form               batched | plain: all pivots in one launch (default),
- *                              or one whole-GPU launch per pivot (FP64)
+ *                              or one cooperative launch per pivot (FP64)
  *         --full-output        also write full-precision, unclamped BC values
  *         --csr-cache          binary CSR cache of the input (read when it
  *                              matches the input file, else written)

@@ -83,12 +83,14 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * half wavefront, two sources sharing its instruction stream; default 2048,
  * 0: off), "autotune" (1, the default: a large batched run of the pivot
  * phase, exact BC or the cluster evaluation on a graph with no tuned
- * configuration yet runs its first tasks as interleaved trial sets, one per
- * candidate -- delta of 4, 2 and 8 mean arc weights, and for views small
- * enough for half-wavefront teams also whole wavefronts -- keeps the
- * candidate with the most traversed edges per millisecond for the rest of
- * the run and for later runs on the graph; an explicit "delta" turns it
- * off; 0: off), "ballot_append" (1: queue and level appends take one
+ * configuration yet tunes on its first tasks -- one launch of one task per
+ * slot with mixed delta candidates (powers of two of the mean arc weight, 8
+ * down to 1/16) locates the region, launches at one delta each walk to the
+ * best neighbour, and for views small enough for half-wavefront teams one
+ * launch of whole wavefronts decides the team form -- and keeps the choice
+ * for the rest of the run and for later runs on the graph; no task runs
+ * twice; an explicit "delta" turns it off; 0: off), "ballot_append" (1:
+ * queue and level appends take one
  * atomic per wavefront -- a ballot of the appending lanes, the first adds
  * their count -- instead of one per lane; same results; 0, the default,
  * because it measured 0.5-1% slower on the 1M-vertex grid).  No counterpart
@@ -112,8 +114,9 @@ int fbc_sssp_sigma(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *src, uint32
 /* ISSBrandesBC::singleSourceBrandes (brandes/DijkstraSSBrandesBC.h:45-81):
  * dep[n] = dependency of every vertex on paths from src (0 for src and for
  * unreached vertices).  One source per call: the plain form.  With
- * FBC_PREC_F64 the source is spread over the whole GPU (one cooperative
- * launch, grid-wide barriers) and the sums follow the reference's successor
+ * FBC_PREC_F64 the source is spread over one cooperative launch (16
+ * workgroups unless fbc_set_tuning's 'slots' says otherwise; grid-wide
+ * barriers) and the sums follow the reference's successor
  * order, so dep equals the reference's bit for bit; FBC_PREC_F32C runs the
  * batched kernel with a single source. */
 int fbc_single_source_dep(fbc_ctx *ctx, const fbc_graph *g, uint32_t src, int prec,
@@ -218,13 +221,15 @@ int fbc_pivot_phase(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
 
 /* The same pivot loop in the plain form, as the reference runs it: one
  * singleSourceBrandes per pivot (ClusteredBrandesBC.h:143-148 calling
- * DijkstraSSBrandesBC.h:45-81), here one cooperative whole-GPU launch per
+ * DijkstraSSBrandesBC.h:45-81), here one cooperative launch per
  * pivot with FP64 sums in the reference's successor order.  Same result
  * contract as fbc_pivot_phase with FBC_PREC_F64; the baseline the batched
  * form is measured against.  fbc_set_tuning's 'slots' sets the workgroups of
- * the grid (0: one per CU).  If the runtime refuses the first cooperative
- * launch of a call for want of device resources (another process holds the
- * device's cooperative-launch resources), this and fbc_single_source_dep
+ * the grid (0: 16, a few more than a thin frontier keeps busy; the grid
+ * barrier of every round and level costs more with a larger grid).  If the
+ * runtime refuses the first cooperative launch of a call for want of device
+ * resources (another process on the device has made cooperative launches:
+ * HIP status 401, hipErrorIllegalState), this and fbc_single_source_dep
  * finish on the batched kernel, with the same values, and fbc_last_counters
  * reports it; any other launch failure is returned as FBC_E_HIP. */
 int fbc_pivot_phase_plain(fbc_ctx *ctx, const fbc_clusters *cl, const uint32_t *pivot,
@@ -260,7 +265,7 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * number of workgroups), the sources whose sigma counts outgrew the packed
  * 48-bit form and were counted again in 64 bits; then, for the plain form
  * (fbc_single_source_dep with FBC_PREC_F64, fbc_pivot_phase_plain), the
- * sources run by cooperative whole-GPU launches, the sources finished on the
+ * sources run by cooperative launches, the sources finished on the
  * batched kernel after a refused first launch, and the HIP status of that
  * refusal (0: none); then the delta and half-team limit of the last run's
  * (main) launch and how they were chosen (0: fixed or set by option, 1:

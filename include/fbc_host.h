@@ -41,7 +41,7 @@ int fbc_kmeans(int k, int P, int nb, const double *vi_len, const int32_t *vi_cnt
 /* How fbc_clustered_bc runs its pivot phase. */
 enum {
     FBC_FORM_BATCHED = 0,      /* every pivot in one persistent launch (fbc_pivot_phase) */
-    FBC_FORM_PLAIN = 1,        /* one whole-GPU launch per pivot, FP64 (fbc_pivot_phase_plain) */
+    FBC_FORM_PLAIN = 1,        /* one cooperative launch per pivot, FP64 (fbc_pivot_phase_plain) */
     FBC_FORM_SELECT_ONLY = 2   /* stop after pivot selection: bc = intra-cluster BC */
 };
 

@@ -15,7 +15,7 @@ make_golden.py) and against properties that hold at any size.
       reference's, dependency rows equal the reference's (digests), Bellman
       and sigma recurrences hold, and the FP32 two-sum pivot phase stays
       within 1e-6 of the FP64 one on a strided sample of the pivots.
-  65,536-vertex road grid: the plain form (one cooperative whole-GPU launch
+  65,536-vertex road grid: the plain form (one cooperative launch
       per pivot) over every pivot of the run, against the batched form.
 """
 import hashlib

@@ -41,7 +41,7 @@ def cfg0(gpu_ctx):
 
 
 def assert_plain_ran(ctx, nsrc):
-    """The last call ran every source on k_plain (cooperative whole-GPU
+    """The last call ran every source on k_plain (cooperative
     launches), none on the batched kernel."""
     c = ctx.last_counters()
     assert c["plain_fallback_sources"] == 0, f"cooperative launch refused (HIP status {c['plain_refusal']})"
@@ -181,7 +181,7 @@ def test_cfg0_pivot_phase(cfg0):
 
 
 def test_cfg0_plain_single_source_bitexact(cfg0, gpu_ctx):
-    """The plain form (one whole-GPU cooperative launch per source, FP64 in
+    """The plain form (one cooperative launch per source, FP64 in
     the reference's successor order) equals the oracle bit for bit."""
     g, og = cfg0
     for s in (0, 17, 2049, g.n - 1):
