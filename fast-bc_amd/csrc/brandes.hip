@@ -36,6 +36,7 @@
 #include "fbc_internal.h"
 
 #include <hip/hip_cooperative_groups.h>
+#include <type_traits>
 
 namespace fbcdev {
 
@@ -235,8 +236,23 @@ struct View {
     const uint2 *rarcs;
     const uint32_t *rrp;
     const uint4 *ell;   /* short rows of the view, or nullptr */
+    const uint4 *lrows; /* the same rows staged in LDS (the Kahn step and the
+                           fold read masked arcs from them), or nullptr */
     uint32_t n;
 };
+
+/* Short row of v (ELL_WIDE in .x when the view's rows are not in LDS or v's
+ * row does not fit), and arc j of it as (target, weight): a short row lists
+ * v's arcs in CSR order. */
+__device__ __forceinline__ uint4 lds_row(const View &V, uint32_t v)
+{
+    return V.lrows ? V.lrows[v] : make_uint4(ELL_WIDE, ELL_NONE, ELL_NONE, ELL_NONE);
+}
+__device__ __forceinline__ uint2 row_arc(const uint4 r, uint32_t v, uint32_t j)
+{
+    const uint32_t e = j == 0 ? r.x : j == 1 ? r.y : j == 2 ? r.z : r.w;
+    return make_uint2(v + (uint32_t)(int32_t)(int16_t)(e & 0xFFFFu), e >> 16);
+}
 
 /* Per-thread work counters, flushed into BrandesParams::stats. */
 struct Counters {
@@ -356,10 +372,11 @@ __device__ __forceinline__ void wave_sync()
 /* One wavefront per source (small views): WAVES sources per workgroup, no
  * workgroup barriers.  Slot state keeps workgroup scope (the wavefront is
  * part of one workgroup). */
-struct TeamWave {
+template <uint32_t LQN>
+struct TeamWaveT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
     static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
-    static constexpr uint32_t LQ = LQ_WAVE;
+    static constexpr uint32_t LQ = LQN, LANES = 64;
     Ctl *ctl;
     Local *loc;
     uint32_t *lq;
@@ -379,10 +396,11 @@ struct TeamWave {
  * they disagree runs each half in turn.  For small views whose rounds and
  * levels hold a few dozen vertices, two sources then share each memory
  * latency of the step. */
-struct TeamHalf {
+template <uint32_t LQN>
+struct TeamHalfT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
     static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
-    static constexpr uint32_t LQ = LQ_HALF;
+    static constexpr uint32_t LQ = LQN, LANES = 32;
     Ctl *ctl;
     Local *loc;
     uint32_t *lq;
@@ -418,10 +436,13 @@ struct TeamGrid {
     __device__ void lsync() const { __syncthreads(); }
 };
 
+using TeamWave = TeamWaveT<LQ_WAVE>;
+using TeamHalf = TeamHalfT<LQ_HALF>;
+
 template <class T> struct IsWG { static constexpr bool value = false; };
 template <> struct IsWG<TeamWG> { static constexpr bool value = true; };
 template <class T> struct IsHalf { static constexpr bool value = false; };
-template <> struct IsHalf<TeamHalf> { static constexpr bool value = true; };
+template <uint32_t N> struct IsHalf<TeamHalfT<N>> { static constexpr bool value = true; };
 
 /* A place in a list counted by *cnt for each calling lane, one atomic per
  * wavefront: the lanes that append -- exactly those executing this call,
@@ -781,11 +802,13 @@ __device__ __forceinline__ void arrive(const Slot S, uint32_t n, const uint32_t 
 /* Push sigma(u) along u's tight arcs, given by its mask (bit j: arc e0 + j),
  * RB at a time. */
 template <bool PACKED, class T>
-__device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint64_t su, uint32_t e0,
+__device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, uint32_t u, uint64_t su, uint32_t e0,
                                                   uint32_t mask, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
                                                   uint32_t *wide, uint32_t *err)
 {
     constexpr int RB = T::RB;
+    const uint4 row = lds_row(V, u);
+    const bool lr = row.x != ELL_WIDE;
     uint32_t m = mask;
     while (m) {
         uint32_t j[RB];
@@ -798,7 +821,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
         }
         uint32_t w[RB];
 #pragma unroll
-        for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
+        for (int k = 0; k < RB; ++k) w[k] = ok[k] ? (lr ? row_arc(row, u, j[k]).x : V.arcs[e0 + j[k]].x) : 0u;
         arrive<PACKED, T, RB>(S, V.n, w, ok, su, lbase, lcnt, lqn, wide, err);
     }
 }
@@ -966,7 +989,7 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
             }
             const uint32_t mask = (uint32_t)y;
             if (PACKED && mask != MASK_NONE) {   /* at most MASK_ARCS arcs: never heavy */
-                push_sigma_masked<PACKED, T>(V, S, su, (uint32_t)(y >> 32), mask, le, lc, lqn, &c.wide, err);
+                push_sigma_masked<PACKED, T>(V, S, u, su, (uint32_t)(y >> 32), mask, le, lc, lqn, &c.wide, err);
                 continue;
             }
             const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
@@ -978,7 +1001,7 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
                 }
             }
             if (mask != MASK_NONE)
-                push_sigma_masked<PACKED, T>(V, S, su, e0, mask, le, lc, lqn, &c.wide, err);
+                push_sigma_masked<PACKED, T>(V, S, u, su, e0, mask, le, lc, lqn, &c.wide, err);
             else
                 push_sigma<PACKED, T>(V, S, dist_of<SC>(S, u), su, e0, e1, 1, le, lc, lqn, &c.wide, err);
         }
@@ -1114,13 +1137,16 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         if (__popc(mask) <= SUCC_REG) {
             /* the successors' (sigma, delta) loads go out together; only the
              * sums follow the order */
+            const uint4 row = lds_row(V, v);
+            const bool lr = row.x != ELL_WIDE;
             uint64_t key[SUCC_REG];
             uint32_t m = mask;
 #pragma unroll
             for (int j = 0; j < SUCC_REG; ++j) {
                 key[j] = 0;
                 if (m) {
-                    const uint2 a = V.arcs[e0 + __builtin_ctz(m)];
+                    const uint32_t aj = (uint32_t)__builtin_ctz(m);
+                    const uint2 a = lr ? row_arc(row, v, aj) : V.arcs[e0 + aj];
                     key[j] = ((uint64_t)(dv + a.y) << 32) | a.x;
                     m &= m - 1;
                 }
@@ -1299,7 +1325,7 @@ __device__ __forceinline__ void flush_counters(const BrandesParams &p, const T &
  * q is its queue index (which trial set it belongs to in a trial launch). */
 template <int MODE, int PREC, bool ORDERED, class T>
 __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, const T &t, uint32_t task,
-                                         uint32_t q, Counters &ct)
+                                         uint32_t q, Counters &ct, const uint4 *ell_view = nullptr)
 {
     View V;
     uint32_t c = 0;
@@ -1309,13 +1335,15 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
         V.n = p.cl_n[c];
         V.rp = p.rp + ro;
         V.rrp = p.rrp + ro;
-        V.ell = p.ell ? p.ell + p.cl_voff[c] : nullptr;
+        /* the cluster's short rows: staged in LDS by the LDS-view kernel */
+        V.ell = ell_view ? ell_view : p.ell ? p.ell + p.cl_voff[c] : nullptr;
     } else {
         V.n = p.n;
         V.rp = p.rp;
         V.rrp = p.rrp;
         V.ell = p.ell;
     }
+    V.lrows = ell_view;
     V.arcs = p.arcs;
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];
@@ -1446,6 +1474,159 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     flush_counters(p, t, ct);
 }
 
+/* ---- the LDS-view kernel: cluster views whose short rows sit in LDS ------- */
+
+/* Per-workgroup LDS of the LDS-view kernel besides the rows: NT teams of
+ * LQN-entry queue heads; the workgroup's current run of tasks (end in the
+ * high 32 bits, next in the low 32; end RUN_BUSY while a team claims the next
+ * run); the staged view's state word: the staged cluster (high 32 bits;
+ * VIEW_NONE while empty or being staged) and the number of teams using it
+ * (low 32 bits). */
+template <int NT, uint32_t LQN>
+struct SharedLds {
+    Ctl ctl[NT];
+    Local loc[NT];
+    uint32_t lq[NT][2 * LQN];
+    unsigned long long run;
+    unsigned long long view;
+};
+constexpr uint32_t VIEW_NONE = 0xFFFFFFFFu;
+constexpr uint32_t RUN_BUSY = 0xFFFFFFFFu;
+
+/* The next task of the workgroup's run (team leader only; ntasks or more
+ * when the queue is empty).  A team that finds the run used up claims the
+ * next one from the device queue: a quarter of the remaining tasks per
+ * workgroup, at most run_max and at least one.  A team that finds another
+ * claiming takes a single task from the device queue instead of waiting:
+ * the two half-wavefront teams of a wavefront run in lockstep, so one must
+ * never wait for the other. */
+__device__ __forceinline__ uint32_t run_next(unsigned long long *run, uint32_t *queue_head, uint32_t ntasks,
+                                             uint32_t run_max)
+{
+    unsigned long long s = __hip_atomic_load(run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+        const uint32_t next = (uint32_t)s, end = (uint32_t)(s >> 32);
+        if (end == RUN_BUSY) return atomicAdd(queue_head, 1u);
+        if (next < end) {
+            if (__hip_atomic_compare_exchange_strong(run, &s, s + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return next;
+            continue;
+        }
+        const unsigned long long busy = (unsigned long long)RUN_BUSY << 32;
+        if (!__hip_atomic_compare_exchange_strong(run, &s, busy, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP))
+            continue;
+        const uint32_t h0 = __hip_atomic_load(queue_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t rem = h0 < ntasks ? ntasks - h0 : 0u;
+        uint32_t len = rem / (4u * gridDim.x);
+        len = len < 1u ? 1u : len > run_max ? run_max : len;
+        const uint32_t h = atomicAdd(queue_head, len);
+        const uint32_t e = h >= ntasks ? ntasks : ntasks - h < len ? ntasks : h + len;
+        const uint32_t b = h >= ntasks ? ntasks : h;
+        /* this team takes b; the run holds [b + 1, e) */
+        __hip_atomic_store(run, ((unsigned long long)e << 32) | (b < e ? b + 1 : e), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        return b;
+    }
+}
+/* queue heads of an LDS-view team: wavefronts 128 entries, half wavefronts 64 */
+constexpr uint32_t LQ_LDS_WAVE = 128, LQ_LDS_HALF = 64;
+/* longest run of tasks a workgroup claims at once */
+constexpr uint32_t LDS_RUN_MAX = 256;
+
+/* The team leader's claim on cluster c's rows in LDS: true when they are
+ * staged and its use is counted; when no team uses the staged view (or none
+ * is staged) it takes the view over and returns *stage = true, the view
+ * marked as being staged.  Otherwise false: the team reads its rows from
+ * HBM. */
+__device__ __forceinline__ bool view_acquire(unsigned long long *view, uint32_t c, bool *stage)
+{
+    *stage = false;
+    unsigned long long s = __hip_atomic_load(view, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+        const uint32_t vc = (uint32_t)(s >> 32), users = (uint32_t)s;
+        unsigned long long want;
+        if (vc == c)
+            want = s + 1;
+        else if (users == 0)
+            want = ((unsigned long long)VIEW_NONE << 32) | 1ull;   /* staging: one user, the stager */
+        else
+            return false;
+        if (__hip_atomic_compare_exchange_strong(view, &s, want, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            *stage = vc != c;
+            return true;
+        }
+    }
+}
+
+/* Cluster evaluation with the cluster's short rows in LDS (dynamic shared
+ * memory, p.lds_rows_max rows).  One workgroup of LWG threads per CU holds
+ * NT teams (wavefronts, or half wavefronts when HALF) -- the occupancy of the
+ * other team kernels -- and one staged view that all its teams may share.
+ * Tasks go out cluster by cluster; a workgroup claims runs of consecutive
+ * tasks (run_next), so its teams work on one cluster for a while, and each
+ * team takes the run's next task when it finishes one.  A task whose
+ * cluster is staged reads its rows (distance phase, Kahn step, ordered fold)
+ * from LDS; when no team of the workgroup uses the staged view, the next
+ * team to take a task of a cluster that fits restages the view to that
+ * cluster; tasks that find the view staged to another cluster in use, or
+ * being staged, read their rows from HBM.  No team waits for another's
+ * task. */
+template <int MODE, int PREC, bool ORDERED, bool HALF>
+__global__ __launch_bounds__(LWG) void k_brandes_lds(BrandesParams p)
+{
+    using T = typename std::conditional<HALF, TeamHalfT<LQ_LDS_HALF>, TeamWaveT<LQ_LDS_WAVE>>::type;
+    constexpr int NT = LWG / (int)T::LANES;
+    static_assert(NT == (HALF ? LDS_TEAMS_HALF : LDS_TEAMS_WAVE), "teams per LDS-view workgroup");
+    extern __shared__ uint4 lds_rows[];
+    __shared__ SharedLds<NT, T::LQ> sh;
+    if (threadIdx.x == 0) {
+        sh.view = (unsigned long long)VIEW_NONE << 32;
+        sh.run = 0;   /* used up: the first team claims a run */
+    }
+    __syncthreads();
+    const uint32_t tm = threadIdx.x / T::LANES;
+    if (blockIdx.x * (uint32_t)NT + tm >= p.nslots) return;   /* no slot behind this team */
+    Slot S;
+    slot_layout(p.nmax, p.scratch + ((uint64_t)blockIdx.x * NT + tm) * p.slot_bytes, &S);
+    S.agg = p.ballot_append != 0;
+    const T t{&sh.ctl[tm], &sh.loc[tm], sh.lq[tm]};
+    if (t.lleader()) sh.loc[tm].te = 0;
+    const int lead = (int)(threadIdx.x & (64u - T::LANES));   /* the team's first lane in its wavefront */
+    Counters ct;
+    for (;;) {
+        uint32_t task = 0;
+        if (t.lleader()) task = run_next(&sh.run, p.queue_head, p.ntasks, LDS_RUN_MAX);
+        task = __shfl(task, lead, 64);
+        if (task >= p.ntasks) break;
+        const uint32_t c = p.task_aux[p.task_base + task];
+        const uint32_t nc = p.cl_n[c];
+        uint32_t mode = 0;   /* 0: HBM rows, 1: staged rows, 2: stage them first */
+        if (nc <= p.lds_rows_max && t.lleader()) {
+            bool stage = false;
+            if (view_acquire(&sh.view, c, &stage)) mode = stage ? 2u : 1u;
+        }
+        mode = __shfl(mode, lead, 64);
+        if (mode == 2) {
+            const uint4 *src = p.ell + p.cl_voff[c];
+            for (uint32_t i = t.lrank(); i < nc; i += T::LANES) lds_rows[i] = src[i];
+            t.lsync();
+            if (t.lleader())
+                __hip_atomic_store(&sh.view, ((unsigned long long)c << 32) | 1ull, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        run_task<MODE, PREC, ORDERED>(p, S, t, p.task_base + task, task, ct, mode ? lds_rows : nullptr);
+        if (mode) {
+            t.lsync();   /* every lane is done with the rows */
+            if (t.lleader())
+                __hip_atomic_fetch_sub(&sh.view, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    flush_counters(p, t, ct);
+}
+
 /* ---- the plain kernel: one source per cooperative launch over the grid ---- */
 
 template <int MODE>
@@ -1463,6 +1644,7 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     V.rp = p.rp;
     V.rrp = p.rrp;
     V.ell = p.ell;
+    V.lrows = nullptr;
     V.arcs = p.arcs;
     V.rarcs = p.rarcs;
     const uint32_t src = p.task_src[task];
@@ -1480,10 +1662,45 @@ __global__ __launch_bounds__(WG) void k_plain(BrandesParams p, uint32_t task)
     flush_counters(p, t, ct);
 }
 
+/* Largest dynamic LDS of an LDS-view workgroup: a workgroup's LDS limit
+ * (the whole 160 KB of a gfx950 CU) less the kernel's static part. */
+template <int MODE, int PREC, bool ORDERED, bool HALF>
+static hipError_t lds_capacity(uint32_t *bytes)
+{
+    const void *f = reinterpret_cast<const void *>(&k_brandes_lds<MODE, PREC, ORDERED, HALF>);
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, f);
+    if (e != hipSuccess) return e;
+    int dev = 0, lds = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+        return e;
+    *bytes = lds > (int)a.sharedSizeBytes ? (uint32_t)(lds - (int)a.sharedSizeBytes) : 0u;
+    return hipSuccess;
+}
+
+template <int MODE, int PREC, bool ORDERED, bool HALF>
+static hipError_t launch_lds(const BrandesParams &p, uint32_t slots, hipStream_t s)
+{
+    const void *f = reinterpret_cast<const void *>(&k_brandes_lds<MODE, PREC, ORDERED, HALF>);
+    constexpr uint32_t NT = HALF ? LDS_TEAMS_HALF : LDS_TEAMS_WAVE;
+    const uint32_t bytes = p.lds_rows_max * (uint32_t)sizeof(uint4);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_brandes_lds<MODE, PREC, ORDERED, HALF>), dim3((slots + NT - 1) / NT), dim3(LWG), bytes, s,
+                       p);
+    return hipGetLastError();
+}
+
 template <int MODE, int PREC, bool ORDERED>
 static hipError_t launch_one(const BrandesParams &p, uint32_t slots, hipStream_t s)
 {
     const int kind = team_kind(p.nmax, p.half_max);
+    if constexpr (MODE == MODE_CLUSTER) {
+        if (p.lds_rows_max)
+            return kind == TEAM_HALF ? launch_lds<MODE, PREC, ORDERED, true>(p, slots, s)
+                                     : launch_lds<MODE, PREC, ORDERED, false>(p, slots, s);
+    }
     if (kind == TEAM_HALF)
         hipLaunchKernelGGL((k_brandes_half<MODE, PREC, ORDERED>), dim3((slots + HALVES - 1) / HALVES), dim3(WG), 0, s,
                            p);
@@ -1514,6 +1731,11 @@ hipError_t launch_brandes(int mode, int prec, bool ordered, const BrandesParams 
     case MODE_CLUSTER: return launch_one<MODE_CLUSTER, 0, true>(p, slots, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t lds_view_capacity(bool half, uint32_t *bytes)
+{
+    return half ? lds_capacity<MODE_CLUSTER, 0, true, true>(bytes) : lds_capacity<MODE_CLUSTER, 0, true, false>(bytes);
 }
 
 static const void *plain_kernel(int mode)

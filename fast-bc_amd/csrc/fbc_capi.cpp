@@ -67,10 +67,17 @@ struct fbc_ctx {
     uint8_t *noalloc = nullptr;   /* this device's address of 'scratch' */
     uint32_t autotune = 1;        /* tune delta / team form on large batched runs */
     uint32_t ballot_append = 0;   /* wavefront-aggregated list appends */
+    /* cluster evaluation on the LDS-view kernel: 0 never, 1 always, 2 when
+     * the clusters whose rows fit in LDS carry at least half of the run's
+     * work (lds_fit_share, set by the cluster evaluation for each launch) and
+     * every team gets 32 tasks or more */
+    uint32_t lds_view = 2;
+    double lds_fit_share = 0;
     /* the last batched run: its delta and half-team limit, and whether they
      * were measured by that run (1), taken from the graph's cache (2) or
      * fixed (0) */
     uint32_t last_delta = 0, last_half_max = 0, last_tuned = 0;
+    uint32_t last_lds = 0;   /* the last batched run used the LDS-view kernel */
     /* the last run met a sigma of 2^53 or more (exact here; the reference's
      * FP64 count would be rounded) */
     uint32_t last_sigma_inexact = 0;
@@ -80,6 +87,7 @@ struct fbc_ctx {
 struct TuneChoice {
     bool set = false;
     uint32_t delta = 0, half_max = 0;
+    bool lds = false;   /* cluster evaluation on the LDS-view kernel */
 };
 
 struct fbc_graph {
@@ -192,14 +200,18 @@ uint32_t auto_delta(const fbc_ctx *ctx, const fbc_graph *g)
 
 /* Make room for 'slots' slots of views up to nmax vertices; returns the slot
  * count actually available. */
-int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out, uint32_t half_max)
+int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *slots_out, uint32_t half_max,
+                   bool lds = false)
 {
     const uint64_t sb = fbcdev::slot_bytes(nmax);
     /* whole-graph slots: 8 per CU (on the 1M-vertex grid's pivot phase 4, 6, 8
      * and 12 per CU took 3277, 3206, 3159 and 3162 ms); a wavefront-per-source
      * workgroup holds 4 slots, 8 such workgroups per CU */
     const int kind = fbcdev::team_kind(nmax, half_max);
-    const uint32_t per_cu = kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 8u;
+    /* the LDS-view kernel: one workgroup of LDS_TEAMS_* teams per CU */
+    const uint32_t lds_teams = kind == fbcdev::TEAM_HALF ? fbcdev::LDS_TEAMS_HALF : fbcdev::LDS_TEAMS_WAVE;
+    const uint32_t per_cu =
+        lds ? lds_teams : kind == fbcdev::TEAM_HALF ? 64u : kind == fbcdev::TEAM_WAVE ? 32u : 8u;
     uint32_t slots = ctx->slots_override ? ctx->slots_override : (uint32_t)ctx->cu_count * per_cu;
     if (slots > want_slots) slots = want_slots;
     if (slots < 1) slots = 1;
@@ -219,7 +231,8 @@ int ensure_scratch(fbc_ctx *ctx, uint32_t nmax, uint32_t want_slots, uint32_t *s
     if ((uint64_t)slots * sb > budget) slots = (uint32_t)std::max<uint64_t>(1, budget / sb);
     /* the wavefront kernels' workgroups address WAVES (HALVES) slots each:
      * hold whole workgroups' worth */
-    const uint32_t unit = kind == fbcdev::TEAM_HALF ? fbcdev::HALVES : kind == fbcdev::TEAM_WAVE ? fbcdev::WAVES : 1;
+    const uint32_t unit =
+        lds ? lds_teams : kind == fbcdev::TEAM_HALF ? fbcdev::HALVES : kind == fbcdev::TEAM_WAVE ? fbcdev::WAVES : 1;
     slots = (slots + unit - 1) / unit * unit;
     if ((size_t)slots * sb > ctx->scratch_bytes) {
         dfree(ctx->scratch);
@@ -271,7 +284,7 @@ int timed_launch(fbc_ctx *ctx, int mode, int prec, bool ordered, BrandesParams q
                  double *te_out)
 {
     uint32_t slots = 0;
-    int rc = ensure_scratch(ctx, q.nmax, q.ntasks, &slots, q.half_max);
+    int rc = ensure_scratch(ctx, q.nmax, q.ntasks, &slots, q.half_max, q.lds_rows_max > 0);
     if (rc) return rc;
     q.scratch = ctx->scratch;
     q.slot_bytes = fbcdev::slot_bytes(q.nmax);
@@ -303,6 +316,23 @@ int timed_launch(fbc_ctx *ctx, int mode, int prec, bool ordered, BrandesParams q
     *err |= e;
     if (ms_out) *ms_out = ms;
     if (te_out) *te_out = (double)te;
+    return 0;
+}
+
+/* Put configuration u on launch parameters r: delta, half-team limit and,
+ * for cluster evaluation, the LDS-view kernel with as many rows as a
+ * workgroup's LDS holds. */
+int apply_choice(fbc_ctx *ctx, BrandesParams &r, const TuneChoice &u)
+{
+    r.delta = u.delta;
+    r.half_max = u.half_max;
+    r.lds_rows_max = 0;
+    if (u.lds) {
+        const bool half = fbcdev::team_kind(r.nmax, r.half_max) == fbcdev::TEAM_HALF;
+        uint32_t bytes = 0;
+        HIP_TRY(ctx, fbcdev::lds_view_capacity(half, &bytes));
+        r.lds_rows_max = std::min<uint32_t>(bytes / (uint32_t)sizeof(uint4), r.nmax);
+    }
     return 0;
 }
 
@@ -339,16 +369,27 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
     TuneChoice def;
     def.delta = auto_delta(ctx, g);
     def.half_max = ctx->half_max;
+    const bool cluster = mode == fbcdev::MODE_CLUSTER;
+    /* the LDS-view kernel: measured on the 1M-vertex grid's clusters (81% of
+     * the work in views of up to ~9,100 vertices, which fit) 1,116 -> 1,091
+     * ms; on the 65K-vertex grid's (8 tasks per team) 8.4 ms either way */
+    const uint32_t lds_teams = fbcdev::team_kind(p.nmax, ctx->half_max) == fbcdev::TEAM_HALF
+                                   ? fbcdev::LDS_TEAMS_HALF
+                                   : fbcdev::LDS_TEAMS_WAVE;
+    def.lds = cluster && (ctx->lds_view == 1 ||
+                          (ctx->lds_view == 2 && ctx->lds_fit_share >= 0.5 &&
+                           (uint64_t)p.ntasks >= 32ull * lds_teams * (uint64_t)ctx->cu_count));
     const int ts = tune_slot(mode);
     const bool tunable = ctx->autotune && !ctx->delta_override && ts >= 0;
     ctx->last_tuned = 0;
     TuneChoice use = def;
     if (tunable && g->tuned[ts].set) {
         use = g->tuned[ts];
+        use.lds = def.lds;   /* a property of this run's views, not of the graph */
         ctx->last_tuned = 2;
     } else if (tunable) {
         uint32_t slots = 0;
-        int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max);
+        int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max, def.lds);
         if (rc) return rc;
         if ((uint64_t)p.ntasks >= 2ull * slots) {
             std::vector<uint32_t> cand;   /* descending */
@@ -362,7 +403,7 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             {
                 BrandesParams q = p;
                 q.ntasks = slots;
-                q.half_max = def.half_max;
+                if ((rc = apply_choice(ctx, q, def))) return rc;
                 q.trial_sets = C;
                 for (uint32_t k = 0; k < C; ++k) q.trial_delta[k] = cand[k];
                 q.trial_stats = ctx->d_trial;
@@ -383,14 +424,14 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
                 }
             }
             /* one launch of n tasks at one configuration; its edges per ms */
-            auto pure = [&](uint32_t delta, uint32_t half_max, uint32_t n, double *rate) -> int {
+            auto pure = [&](const TuneChoice &u, uint32_t n, double *rate) -> int {
                 BrandesParams r = p;
                 r.task_base = (uint32_t)done;
                 r.ntasks = n;
-                r.delta = delta;
-                r.half_max = half_max;
+                int e = apply_choice(ctx, r, u);
+                if (e) return e;
                 double ms = 0, te = 0;
-                const int e = timed_launch(ctx, mode, prec, ordered, r, err, &ms, &te);
+                e = timed_launch(ctx, mode, prec, ordered, r, err, &ms, &te);
                 if (e) return e;
                 done += n;
                 *rate = te / std::max(ms, 1e-3);
@@ -401,13 +442,15 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             use.delta = cand[bi];
             double best = -1;
             if (room(slots)) {
-                if ((rc = pure(cand[bi], def.half_max, slots, &best))) return rc;
+                if ((rc = pure(use, slots, &best))) return rc;
                 uint32_t at = bi;
                 for (int dir : {+1, -1}) {   /* +1: smaller deltas */
                     bool moved = false;
                     for (int k = (int)at + dir; k >= 0 && k < (int)C && room(slots); k += dir) {
+                        TuneChoice u = use;
+                        u.delta = cand[(uint32_t)k];
                         double r = 0;
-                        if ((rc = pure(cand[(uint32_t)k], def.half_max, slots, &r))) return rc;
+                        if ((rc = pure(u, slots, &r))) return rc;
                         if (r <= best) break;
                         best = r;
                         at = (uint32_t)k;
@@ -420,11 +463,16 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             /* 3. team form */
             if (best > 0 && fbcdev::team_kind(p.nmax, def.half_max) == fbcdev::TEAM_HALF) {
                 uint32_t wave_slots = 0;
-                if ((rc = ensure_scratch(ctx, p.nmax, p.ntasks, &wave_slots, 0))) return rc;
+                if ((rc = ensure_scratch(ctx, p.nmax, p.ntasks, &wave_slots, 0, use.lds))) return rc;
                 if (room(wave_slots)) {
+                    TuneChoice u = use;
+                    u.half_max = 0;
                     double r = 0;
-                    if ((rc = pure(use.delta, 0, wave_slots, &r))) return rc;
-                    if (r > best) use.half_max = 0;
+                    if ((rc = pure(u, wave_slots, &r))) return rc;
+                    if (r > best) {
+                        use.half_max = 0;
+                        best = r;
+                    }
                 }
             }
             use.set = true;
@@ -432,19 +480,19 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             ctx->last_tuned = 1;
             ctx->last_delta = use.delta;
             ctx->last_half_max = use.half_max;
+            ctx->last_lds = use.lds;
             BrandesParams r = p;
             r.task_base = (uint32_t)done;
             r.ntasks = p.ntasks - (uint32_t)done;
-            r.delta = use.delta;
-            r.half_max = use.half_max;
+            if ((rc = apply_choice(ctx, r, use))) return rc;
             return r.ntasks ? timed_launch(ctx, mode, prec, ordered, r, err, nullptr, nullptr) : 0;
         }
     }
     ctx->last_delta = use.delta;
     ctx->last_half_max = use.half_max;
+    ctx->last_lds = use.lds;
     BrandesParams q = p;
-    q.delta = use.delta;
-    q.half_max = use.half_max;
+    if (int rc = apply_choice(ctx, q, use)) return rc;
     return timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr);
 }
 
@@ -637,6 +685,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "half_team_max")) ctx->half_max = std::min<uint32_t>(v, fbcdev::WAVE_TEAM_MAX_N);
     else if (!std::strcmp(name, "autotune")) ctx->autotune = v ? 1 : 0;
     else if (!std::strcmp(name, "ballot_append")) ctx->ballot_append = v ? 1 : 0;
+    else if (!std::strcmp(name, "lds_view")) ctx->lds_view = std::min<uint32_t>(v, 2);
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
@@ -1345,7 +1394,25 @@ static int cluster_eval(fbc_ctx *ctx, fbc_clusters *cl, double *intra_bc, uint32
             p.dmat = d_dmat;
             p.vi_len = d_vil;
             p.vi_cnt = d_vic;
+            /* the share of the chunk's work (n_c * m_c) in views whose short
+             * rows fit in an LDS-view workgroup */
+            {
+                const bool half = fbcdev::team_kind(cl->nmax, ctx->half_max) == fbcdev::TEAM_HALF;
+                uint32_t bytes = 0;
+                if ((e = fbcdev::lds_view_capacity(half, &bytes)) != hipSuccess) {
+                    rc = hip_fail(ctx, e, "LDS capacity");
+                    break;
+                }
+                double fit = 0, all = 0;
+                for (uint32_t c : order) {
+                    const double wk = (double)cl->n[c] * (double)cl->edges[c];
+                    all += wk;
+                    if ((uint64_t)cl->n[c] * sizeof(uint4) <= bytes) fit += wk;
+                }
+                ctx->lds_fit_share = all > 0 ? fit / all : 0.0;
+            }
             rc = run(ctx, g, fbcdev::MODE_CLUSTER, 0, true, p);
+            ctx->lds_fit_share = 0;
             if (rc) break;
             ms += ctx->last_ms;
             te += ctx->last_te;
@@ -1958,7 +2025,7 @@ int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count)
     if (!ctx || !counters || count < 0) return FBC_E_ARG;
     const uint64_t host[] = {ctx->plain_sources, ctx->plain_fallback_sources, ctx->plain_refusal,
                              ctx->last_delta,    ctx->last_half_max,           ctx->last_tuned,
-                             ctx->last_sigma_inexact};
+                             ctx->last_sigma_inexact, ctx->last_lds};
     constexpr int NH = (int)(sizeof(host) / sizeof(host[0]));
     int i = 0;
     for (; i < count && i < fbcdev::STAT_COUNT; ++i) counters[i] = ctx->last_counters[i];

@@ -96,6 +96,9 @@ struct BrandesParams {
     uint32_t nslots;          /* slots the scratch holds (the wavefront kernels check their slot against it) */
     uint32_t half_max;        /* views up to this many vertices: half-wavefront teams */
     uint32_t ballot_append;   /* 1: queue and level appends take one atomic per wavefront */
+    /* cluster evaluation on the LDS-view kernel (lds_rows_max > 0): views of
+     * up to lds_rows_max vertices can have their short rows staged in LDS */
+    uint32_t lds_rows_max;
 };
 
 /* Largest near queue or level the batched kernel sorts (LDS words). */
@@ -133,6 +136,11 @@ inline bool wave_team(uint32_t nmax) { return nmax <= WAVE_TEAM_MAX_N; }
 constexpr uint32_t HALF_TEAM_MAX_N = 2048;
 constexpr int HALVES = WG / 32;            /* slots of a half-wavefront-per-source workgroup */
 enum TeamKind { TEAM_WG = 0, TEAM_WAVE = 1, TEAM_HALF = 2 };
+/* The LDS-view cluster kernel: one workgroup of LWG threads per CU (its LDS
+ * holds one cluster's short rows), as many wavefront or half-wavefront teams
+ * as the other kernels keep resident per CU. */
+constexpr int LWG = 1024;
+constexpr uint32_t LDS_TEAMS_WAVE = LWG / 64, LDS_TEAMS_HALF = LWG / 32;
 inline int team_kind(uint32_t nmax, uint32_t half_max)
 {
     return nmax <= half_max ? TEAM_HALF : wave_team(nmax) ? TEAM_WAVE : TEAM_WG;
@@ -155,6 +163,9 @@ uint64_t slot_bytes(uint32_t nmax);
 hipError_t plain_max_blocks(int mode, int cu_count, uint32_t *blocks);
 hipError_t launch_plain(int mode, const BrandesParams &p, uint32_t task, uint32_t blocks,
                         hipStream_t stream);
+
+/* Dynamic LDS an LDS-view workgroup can hold (bytes; rows of 16 bytes). */
+hipError_t lds_view_capacity(bool half, uint32_t *bytes);
 
 /* Launch the persistent kernel (grid = slots workgroups) on 'stream'. */
 hipError_t launch_brandes(int mode, int prec, bool ordered, const BrandesParams &p,

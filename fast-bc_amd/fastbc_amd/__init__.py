@@ -31,12 +31,13 @@ EXPORTS = (
 
 # Every symbol include/fbc_host.h declares.
 HOST_EXPORTS = ("fbc_louvain", "fbc_select_pivots", "fbc_kmeans", "fbc_clustered_bc", "fbc_clustered_bc_partition",
-                "fbc_exact_bc_arcs", "fbc_host_release", "fbc_host_last_error", "fbc_host_last_tuning")
+                "fbc_exact_bc_arcs", "fbc_host_release", "fbc_host_last_error", "fbc_host_last_tuning",
+                "fbc_host_last_tuning_ex")
 
 COUNTER_NAMES = ("rounds", "windows", "visits", "relax", "improve", "farpush", "levels",
                  "ns_dist", "ns_sigma", "ns_dep", "sigma_wide",
                  "plain_sources", "plain_fallback_sources", "plain_refusal",
-                 "delta", "half_team_max", "tuned", "sigma_inexact")
+                 "delta", "half_team_max", "tuned", "sigma_inexact", "lds_view")
 
 ERRORS = {
     -1: "FBC_E_ARG", -2: "FBC_E_HIP", -3: "FBC_E_OOM", -4: "FBC_E_DEVICE",
@@ -120,6 +121,7 @@ def lib():
         L.fbc_host_release.argtypes = []
         L.fbc_host_release.restype = None
         L.fbc_host_last_tuning.argtypes = [_u64p]
+        L.fbc_host_last_tuning_ex.argtypes = [_u64p, ctypes.c_int]
         L.fbc_host_last_error.argtypes = []
         L.fbc_host_last_error.restype = ctypes.c_char_p
         _lib = L
@@ -160,7 +162,8 @@ class Context:
 
     def set_option(self, name, value):
         """Named tuning option (fbc_set_option): "delta", "slots", "frontier_sort",
-        "relabel", "scratch_mb", "half_team_max", "class_key_bits", "autotune", "ballot_append"."""
+        "relabel", "scratch_mb", "half_team_max", "class_key_bits", "autotune", "ballot_append",
+        "lds_view"."""
         self.check(lib().fbc_set_option(self.h, name.encode(), int(value)))
 
     def last_stats(self):
@@ -476,13 +479,13 @@ def clustered_bc(src, dst, w, seeds, precision=0.01, kfrac=0.0, prec=PREC_F64, d
                                                len(seeds), precision, kfrac, prec, form, bc, comm, intra, piv, pw,
                                                pc, ph),
                         "fbc_clustered_bc")
-    tun = np.zeros(6, np.uint64)
-    lib().fbc_host_last_tuning(tun)
+    tun = np.zeros(8, np.uint64)
+    lib().fbc_host_last_tuning_ex(tun, len(tun))
     t = [int(x) for x in tun]
     return dict(bc=bc, comm=comm, intra=intra, pivots=piv[:k], weights=pw[:k], pivot_cluster=pc[:k],
                 phases=dict(zip(PHASES, ph.tolist())),
-                tuning={"cluster": dict(zip(("delta", "half_team_max", "how"), t[:3])),
-                        "pivot": dict(zip(("delta", "half_team_max", "how"), t[3:]))})
+                tuning={"cluster": dict(zip(("delta", "half_team_max", "how", "lds_view"), t[:4])),
+                        "pivot": dict(zip(("delta", "half_team_max", "how"), t[4:7]))})
 
 
 def host_release():

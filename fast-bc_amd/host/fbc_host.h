@@ -113,7 +113,8 @@ struct PhaseTimes {
     size_t clusters = 0, bordered = 0, pivot_count = 0;
     /* the autotuner's choice for the cluster and pivot launches: delta,
      * half-team limit, and how chosen (fbc_last_counters' last three) */
-    uint64_t cluster_tuning[3] = {0, 0, 0}, pivot_tuning[3] = {0, 0, 0};
+    /* delta, half-team limit, how chosen, LDS-view kernel (cluster launch) */
+    uint64_t cluster_tuning[4] = {0, 0, 0, 0}, pivot_tuning[4] = {0, 0, 0, 0};
 };
 
 struct ClusteredOptions {

@@ -53,7 +53,7 @@ struct Resident {
     std::vector<int32_t> src, dst;
     std::vector<double> w;
     std::shared_ptr<Graph> graph;
-    uint64_t tuning[6] = {0, 0, 0, 0, 0, 0};   /* of the last clustered run */
+    uint64_t tuning[8] = {};   /* of the last clustered run: cluster launch, then pivot launch */
 };
 
 Resident &resident()
@@ -128,8 +128,8 @@ int clustered_entry(int device, uint64_t m, const int32_t *src, const int32_t *d
         if (std::getenv("FBC_PROGRESS"))   /* progress lines on stderr */
             opt.progress = [](int, const std::string &m) { std::fprintf(stderr, "[fbc] %s\n", m.c_str()); };
         ClusteredRun r = clustered_bc(dev, g, opt);
-        std::copy(r.t.cluster_tuning, r.t.cluster_tuning + 3, resident().tuning);
-        std::copy(r.t.pivot_tuning, r.t.pivot_tuning + 3, resident().tuning + 3);
+        std::copy(r.t.cluster_tuning, r.t.cluster_tuning + 4, resident().tuning);
+        std::copy(r.t.pivot_tuning, r.t.pivot_tuning + 4, resident().tuning + 4);
         std::memcpy(bc, r.bc.data(), r.bc.size() * sizeof(double));
         if (comm_of)
             for (size_t i = 0; i < r.comm.size(); ++i) comm_of[i] = r.comm[i];
@@ -256,8 +256,18 @@ int fbc_host_last_tuning(uint64_t *out)
 {
     if (!out) return FBC_E_ARG;
     std::lock_guard<std::mutex> lk(resident().mu);
-    std::copy(resident().tuning, resident().tuning + 6, out);
+    std::copy(resident().tuning, resident().tuning + 3, out);
+    std::copy(resident().tuning + 4, resident().tuning + 7, out + 3);
     return 0;
+}
+
+int fbc_host_last_tuning_ex(uint64_t *out, int count)
+{
+    if (!out || count < 0) return FBC_E_ARG;
+    std::lock_guard<std::mutex> lk(resident().mu);
+    const int k = count < 8 ? count : 8;
+    std::copy(resident().tuning, resident().tuning + k, out);
+    return k;
 }
 
 } // extern "C"

@@ -30,7 +30,11 @@
  */
 #include "fbc_host.h"
 
+#include <sys/mman.h>
+
 #include <algorithm>
+#include <cstdlib>
+#include <new>
 #include <numeric>
 #include <random>
 #include <set>
@@ -38,6 +42,43 @@
 namespace fbchost {
 
 namespace {
+
+/* Arrays of a pass (link runs, node records, community sums, the node order)
+ * are read at random over hundreds of MB, so with 4 KiB pages nearly every
+ * step of the chain also misses the TLB.  Arrays of 2 MiB or more are placed
+ * on 2 MiB boundaries and offered to transparent huge pages
+ * (MADV_HUGEPAGE; where the kernel declines, they stay ordinary pages). */
+template <class T>
+struct HugeAlloc {
+    using value_type = T;
+    static constexpr size_t HUGE = (size_t)2 << 20;
+    HugeAlloc() = default;
+    template <class U>
+    HugeAlloc(const HugeAlloc<U> &) {}
+    T *allocate(size_t n)
+    {
+        const size_t bytes = n * sizeof(T);
+        if (bytes < HUGE) return static_cast<T *>(::operator new(bytes));
+        const size_t sz = (bytes + HUGE - 1) / HUGE * HUGE;
+        void *p = std::aligned_alloc(HUGE, sz);
+        if (!p) throw std::bad_alloc();
+        (void)madvise(p, sz, MADV_HUGEPAGE);
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n)
+    {
+        if (n * sizeof(T) < HUGE)
+            ::operator delete(p);
+        else
+            std::free(p);
+    }
+};
+template <class T, class U>
+bool operator==(const HugeAlloc<T> &, const HugeAlloc<U> &) { return true; }
+template <class T, class U>
+bool operator!=(const HugeAlloc<T> &, const HugeAlloc<U> &) { return false; }
+template <class T>
+using hvec = std::vector<T, HugeAlloc<T>>;
 
 struct LGraph {
     unsigned nodes = 0;
@@ -111,8 +152,8 @@ struct CommTot {
 
 /* The packed links of one level, shared by its instances. */
 struct LevelLinks {
-    std::vector<NodeRec> rec;
-    std::vector<Link> adj;
+    hvec<NodeRec> rec;
+    hvec<Link> adj;
 
     explicit LevelLinks(const LGraph &gr)
     {
@@ -140,14 +181,14 @@ struct Part {
     const LGraph *g;
     int size;
     double min_mod;
-    const std::vector<NodeRec> &rec;
-    const std::vector<Link> &adj;
-    std::vector<double> nw;
-    std::vector<unsigned> npos;
+    const hvec<NodeRec> &rec;
+    const hvec<Link> &adj;
+    hvec<double> nw;
+    hvec<unsigned> npos;
     unsigned nlast = 0;
-    std::vector<int> n2c;
-    std::vector<double> woutc;
-    std::vector<CommTot> tot;
+    hvec<int> n2c;
+    hvec<double> woutc;
+    hvec<CommTot> tot;
 
     Part(const LGraph &gr, const LevelLinks &ll, double minm)
         : g(&gr), size((int)gr.nodes), min_mod(minm), rec(ll.rec), adj(ll.adj)
@@ -228,7 +269,7 @@ struct Part {
      * its links, then its neighbours' communities, then their sums.  A
      * prefetch never changes a value; a community that changes before the
      * node is reached only costs a miss. */
-    void prefetch(const std::vector<int> &order, int t) const
+    void prefetch(const hvec<int> &order, int t) const
     {
         constexpr int A1 = 24, A2 = 12, A3 = 6, A4 = 3;
         if (t + A1 < size) __builtin_prefetch(&rec[order[t + A1]]);
@@ -260,7 +301,7 @@ struct Part {
 
     bool one_level(std::mt19937 &eng)
     {
-        std::vector<int> order((size_t)size);
+        hvec<int> order((size_t)size);
         std::iota(order.begin(), order.end(), 0);
         std::shuffle(order.begin(), order.end(), eng);
         bool improvement = false;
@@ -429,7 +470,7 @@ std::vector<std::vector<int>> louvain(const Graph &g, const std::vector<uint64_t
         std::vector<int> renumber;
         LGraph next = parts[best].aggregate(renumber);
         /* renumber_communities (LouvainGraphPartition.h:27-46) */
-        const std::vector<int> &lvl = parts[best].n2c;
+        const hvec<int> &lvl = parts[best].n2c;
         for (size_t i = 0; i < n2c.size(); ++i) n2c[i] = renumber[lvl[n2c[i]]];
         lg = std::move(next);
     } while (improvement);

@@ -337,12 +337,15 @@ std::vector<double> single_source_dep(Device &dev, const Graph &g, int src, int 
     return dep;
 }
 
-/* delta, half-team limit and how they were chosen, for the last launch */
-static void last_tuning(Device &dev, uint64_t out[3])
+/* delta, half-team limit and how they were chosen, and whether the
+ * LDS-view kernel ran, for the last launch (fbc_last_counters entries 14-16
+ * and 18) */
+static void last_tuning(Device &dev, uint64_t out[4])
 {
     uint64_t c[32] = {};
     const int k = fbc_last_counters(dev.ctx(), c, 32);
     for (int i = 0; i < 3; ++i) out[i] = k >= 17 ? c[14 + i] : 0;
+    out[3] = k >= 19 ? c[18] : 0;
 }
 
 ClusteredRun clustered_bc(Device &dev, const Graph &g, const ClusteredOptions &opt)

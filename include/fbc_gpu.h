@@ -93,8 +93,12 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * queue and level appends take one
  * atomic per wavefront -- a ballot of the appending lanes, the first adds
  * their count -- instead of one per lane; same results; 0, the default,
- * because it measured 0.5-1% slower on the 1M-vertex grid).  No counterpart
- * in the reference. */
+ * because it measured 0.5-1% slower on the 1M-vertex grid), "lds_view"
+ * (cluster evaluation with each cluster's short rows staged in LDS, shared by
+ * the 16 wavefront -- or 32 half-wavefront -- teams of a 1024-thread
+ * workgroup, one workgroup per CU; clusters too large for the LDS read their
+ * rows from HBM: 0 never, 1 always, 2, the default: when the autotuner
+ * measures it faster on the graph).  No counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].
@@ -272,7 +276,8 @@ int fbc_last_stats(const fbc_ctx *ctx, double *kernel_ms, double *traversed_edge
  * measured by the autotuner in this run, 2: the graph's tuned choice from an
  * earlier run); then 1 when the last run met a shortest-path count of 2^53
  * or more (exact here, where the reference's FP64 count is rounded; counts
- * of 2^64 or more return FBC_E_SIGMA_OVERFLOW).  Fills at most 'count'
+ * of 2^64 or more return FBC_E_SIGMA_OVERFLOW); then 1 when the last
+ * batched cluster evaluation ran on the LDS-view kernel.  Fills at most 'count'
  * entries, returns how many it wrote.  Instrumentation only; no counterpart
  * in the reference. */
 int fbc_last_counters(const fbc_ctx *ctx, uint64_t *counters, int count);

@@ -82,6 +82,13 @@ int fbc_exact_bc_arcs(int device, uint64_t m, const int32_t *src, const int32_t 
  * fbc_set_option "autotune").  No counterpart in the reference. */
 int fbc_host_last_tuning(uint64_t *out);
 
+/* The same with the cluster launch's kernel form: out[0..3] = {delta,
+ * half-team limit, how chosen, 1 when the cluster evaluation ran on the
+ * LDS-view kernel} for the cluster launch, out[4..7] the same for the pivot
+ * launch (its LDS entry is always 0).  Writes at most 'count' entries and
+ * returns how many.  No counterpart in the reference. */
+int fbc_host_last_tuning_ex(uint64_t *out, int count);
+
 /* Message of the calling thread's last failed call above (empty when none
  * failed yet).  No counterpart in the reference, whose errors are
  * exceptions. */

@@ -65,3 +65,39 @@ def test_autotune_cluster_eval_identical(gpu_ctx):
           f"half-team limit {c1['half_team_max']}", flush=True)
     assert c1["tuned"] == 1
     assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_lds_view_cluster_eval_identical(gpu_ctx, cfg):
+    """Cluster evaluation with the clusters' short rows staged in LDS (option
+    lds_view=1) equals the HBM-row kernels bit for bit: intra BC, VertexInfo
+    lengths and counts.  cfg1's views (up to ~1,200 vertices) run on
+    half-wavefront teams and all fit; cfg2's (1,159 to 9,870 vertices) run on
+    wavefront teams and the largest read their rows from HBM."""
+    n, (s, d, w) = F.generate_config(cfg)
+    comm, _ = F.louvain(s, d, w, [1, 2, 3, 4], 0.01)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    cl = g.clusters(comm_of=comm)
+    gpu_ctx.set_option("autotune", 0)
+    try:
+        gpu_ctx.set_option("lds_view", 0)
+        bc0, L0, C0 = cl.evaluate()
+        ms0, _ = gpu_ctx.last_stats()
+        assert gpu_ctx.last_counters()["lds_view"] == 0
+        gpu_ctx.set_option("lds_view", 1)
+        bc1, L1, C1 = cl.evaluate()
+        ms1, _ = gpu_ctx.last_stats()
+        assert gpu_ctx.last_counters()["lds_view"] == 1
+        # the default rule: LDS rows when the views that fit carry half the
+        # work and every team gets 32 tasks (cfg2: 81% fit, 1M tasks; cfg1:
+        # 65K tasks for 8,192 half-wavefront teams)
+        gpu_ctx.set_option("lds_view", 2)
+        bc2, L2, C2 = cl.evaluate()
+        assert gpu_ctx.last_counters()["lds_view"] == (1 if cfg == 2 else 0)
+    finally:
+        gpu_ctx.set_option("lds_view", 2)
+        gpu_ctx.set_option("autotune", 1)
+    print(f"cfg{cfg}: cluster launch {ms0:.1f} ms (HBM rows) vs {ms1:.1f} ms (LDS rows), views up to "
+          f"{int(np.bincount(comm).max())} vertices", flush=True)
+    assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
+    assert np.array_equal(bc2, bc0) and np.array_equal(L2, L0) and np.array_equal(C2, C0)
