@@ -56,15 +56,26 @@ def test_autotune_cluster_eval_identical(gpu_ctx):
     try:
         bc1, L1, C1 = cl.evaluate()
         c1 = gpu_ctx.last_counters()
+        # tuned again on the LDS-view kernel: its trial launches (mixed-delta
+        # sets, then one delta each) run tasks whose views are staged in LDS
+        g2 = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+        cl2 = g2.clusters(comm_of=comm)
+        gpu_ctx.set_option("lds_view", 1)
+        bc2, L2, C2 = cl2.evaluate()
+        c2 = gpu_ctx.last_counters()
+        gpu_ctx.set_option("lds_view", 2)
         gpu_ctx.set_option("autotune", 0)
         bc0, L0, C0 = cl.evaluate()
     finally:
+        gpu_ctx.set_option("lds_view", 2)
         gpu_ctx.set_option("autotune", 1)
         gpu_ctx.set_tuning(0, 0)
     print(f"cluster views up to {int(np.bincount(comm).max())} vertices: tuned delta {c1['delta']}, "
-          f"half-team limit {c1['half_team_max']}", flush=True)
-    assert c1["tuned"] == 1
+          f"half-team limit {c1['half_team_max']}; on the LDS view: delta {c2['delta']}, "
+          f"half-team limit {c2['half_team_max']}", flush=True)
+    assert c1["tuned"] == 1 and c2["tuned"] == 1 and c2["lds_view"] == 1
     assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
+    assert np.array_equal(bc2, bc0) and np.array_equal(L2, L0) and np.array_equal(C2, C0)
 
 
 @pytest.mark.parametrize("cfg", [1, 2])
