@@ -61,8 +61,12 @@ constexpr uint64_t PK_SIGMA = PK_ONE - 1;      /* packed: the sigma bits */
 constexpr uint32_t PK_CMAX = 0xFFFFu;          /* packed: largest count */
 constexpr int HEAVY_CAP = 64;
 /* Per team: RB arcs of one vertex have their loads and atomics issued
- * together; the ordered fold keeps up to SUCC successors in registers.  The
- * wavefront team takes smaller blocks to stay at 4 wavefronts per SIMD. */
+ * together; the ordered fold keeps up to SUCC successors in registers; the
+ * masked Kahn step and fold take MB successors per block (a masked row has
+ * one or two as a rule, and a block's unused slots still load, to keep its
+ * loads free of branches: 2 for workgroup teams, where wider blocks cost the
+ * pivot launch 1.3%, 4 for wavefront teams).  The wavefront team takes
+ * smaller blocks to stay at 4 wavefronts per SIMD. */
 
 /* ---- memory helpers ------------------------------------------------------ */
 
@@ -343,7 +347,7 @@ __device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
 /* One workgroup per source (batched kernel). */
 struct TeamWG {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
+    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1, MB = 2;
     static constexpr uint32_t LQ = LQ_WG;
     Ctl *ctl;
     Local *loc;
@@ -375,7 +379,7 @@ __device__ __forceinline__ void wave_sync()
 template <uint32_t LQN>
 struct TeamWaveT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1, MB = 4;
     static constexpr uint32_t LQ = LQN, LANES = 64;
     Ctl *ctl;
     Local *loc;
@@ -399,7 +403,7 @@ struct TeamWaveT {
 template <uint32_t LQN>
 struct TeamHalfT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1, MB = 4;
     static constexpr uint32_t LQ = LQN, LANES = 32;
     Ctl *ctl;
     Local *loc;
@@ -420,7 +424,7 @@ struct TeamHalfT {
  * resident, so the grid barrier always completes). */
 struct TeamGrid {
     static constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
-    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1;
+    static constexpr int RB = 8, SUCC = 8, SWEEP_U = 1, MB = 2;
     static constexpr uint32_t LQ = 0;   /* the grid shares no LDS */
     Ctl *ctl;
     Local *loc;
@@ -554,8 +558,14 @@ __device__ __forceinline__ void relax_arcs(const Slot S, uint32_t du, const uint
             }
         }
     }
+    /* every load of the block is issued, an arc not taken reading key[0]
+     * (always a vertex), so no branch separates them and they are in flight
+     * together; its nk (all ones) then never passes the filter */
+    uint32_t la[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) cur[j] = nk[j] != ~uint64_t(0) ? ald64<SC>(&S.key[to[j]]) : 0;
+    for (int j = 0; j < NB; ++j) la[j] = nk[j] != ~uint64_t(0) ? to[j] : 0u;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) cur[j] = ald64<SC>(&S.key[la[j]]);
 #pragma unroll
     for (int j = 0; j < NB; ++j) cur[j] = nk[j] < cur[j] ? amin64<SC>(&S.key[to[j]], nk[j]) : 0;
 #pragma unroll
@@ -591,8 +601,10 @@ __device__ __forceinline__ void relax_vertex(const View &V, const Slot S, uint32
         bool ok[RB];
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
+            /* past the row's end: its last arc again (a valid index, e1 > eb),
+             * so the block's loads carry no branch and go out together */
             ok[j] = eb + j < e1;
-            a[j] = ok[j] ? V.arcs[eb + j] : make_uint2(0u, 0u);
+            a[j] = V.arcs[ok[j] ? eb + j : e1 - 1];
         }
         relax_arcs<T, RB>(S, du, a, ok, thr, nxt, win, lnext, qnext, qn_cnt, far, far_cnt, far_min, err, ct);
     }
@@ -657,21 +669,22 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
                 const uint32_t du = dist_of<SC>(S, u);
                 if (V.ell) {
                     /* a short row: its arcs in one 16-byte load, no row starts */
+                    /* relaxed whole and before the test for a wide row (which
+                     * has no arc to take: ELL_WIDE, then ELL_NONE), so the
+                     * row is one 16-byte load rather than a word, a test and
+                     * a dependent load of the rest */
                     const uint4 r = V.ell[u];
-                    if (r.x != ELL_WIDE) {
-                        const uint32_t e[ELL_W] = {r.x, r.y, r.z, r.w};
-                        uint2 a[ELL_W];
-                        bool ok[ELL_W];
+                    const uint32_t e[ELL_W] = {r.x, r.y, r.z, r.w};
+                    uint2 a[ELL_W];
+                    bool ok[ELL_W];
 #pragma unroll
-                        for (int j = 0; j < (int)ELL_W; ++j) {
-                            ok[j] = e[j] != ELL_NONE;
-                            a[j] = make_uint2(u + (uint32_t)(int32_t)(int16_t)(e[j] & 0xFFFFu), e[j] >> 16);
-                            ct.c[STAT_RELAX] += ok[j];
-                        }
-                        relax_arcs<T, ELL_W>(S, du, a, ok, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err,
-                                             ct);
-                        continue;
+                    for (int j = 0; j < (int)ELL_W; ++j) {
+                        ok[j] = e[j] < ELL_WIDE;
+                        a[j] = make_uint2(u + (uint32_t)(int32_t)(int16_t)(e[j] & 0xFFFFu), e[j] >> 16);
+                        ct.c[STAT_RELAX] += ok[j];
                     }
+                    relax_arcs<T, ELL_W>(S, du, a, ok, thr, nxt, win, lqb, qb, qn_cnt, fa, far_cnt, far_min, err, ct);
+                    if (r.x != ELL_WIDE) continue;
                 }
                 const uint32_t e0 = V.rp[u], e1 = V.rp[u + 1];
                 if (e1 - e0 > (uint32_t)HEAVY_DEG) {
@@ -806,7 +819,7 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
                                                   uint32_t mask, uint32_t lbase, uint32_t *lcnt, uint32_t *lqn,
                                                   uint32_t *wide, uint32_t *err)
 {
-    constexpr int RB = T::RB;
+    constexpr int RB = T::MB;   /* unused slots load arc e0 again: no branches */
     const uint4 row = lds_row(V, u);
     const bool lr = row.x != ELL_WIDE;
     uint32_t m = mask;
@@ -821,7 +834,8 @@ __device__ __forceinline__ void push_sigma_masked(const View &V, const Slot S, u
         }
         uint32_t w[RB];
 #pragma unroll
-        for (int k = 0; k < RB; ++k) w[k] = ok[k] ? (lr ? row_arc(row, u, j[k]).x : V.arcs[e0 + j[k]].x) : 0u;
+        for (int k = 0; k < RB; ++k)   /* j[k] = 0 when not ok: arc e0 exists (mask != 0) */
+            w[k] = lr ? row_arc(row, u, j[k]).x : V.arcs[e0 + j[k]].x;
         arrive<PACKED, T, RB>(S, V.n, w, ok, su, lbase, lcnt, lqn, wide, err);
     }
 }
@@ -838,12 +852,14 @@ __device__ __forceinline__ void push_sigma(const View &V, const Slot S, uint32_t
     for (uint32_t eb = e0; eb < e1; eb += RB * step) {
         uint2 a[RB];
 #pragma unroll
-        for (int j = 0; j < RB; ++j) a[j] = eb + j * step < e1 ? V.arcs[eb + j * step] : make_uint2(0u, 0u);
+        for (int j = 0; j < RB; ++j) a[j] = V.arcs[eb + j * step < e1 ? eb + j * step : eb];   /* eb < e1: valid */
         bool tight[RB];
-        uint32_t w[RB];
+        uint32_t w[RB], dw[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) dw[j] = dist_of<SC>(S, a[j].x);   /* no branch: in flight together */
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
-            tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dist_of<SC>(S, a[j].x);
+            tight[j] = eb + j * step < e1 && (uint64_t)du + a[j].y == dw[j];
             w[j] = a[j].x;
         }
         arrive<PACKED, T, RB>(S, V.n, w, tight, su, lbase, lcnt, lqn, wide, err);
@@ -870,82 +886,64 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
     const uint32_t r0 = t.rank(), R = t.size();
     Ctl &c = *t.ctl;
     unsigned long long te = 0;
-    /* SU vertices per thread and step, their out- and in-arcs SB at a time:
-     * every load of a step is independent of the others, so they are in
+    /* one vertex per thread and step, its out- and in-arcs SB at a time.
+     * The loads of a block carry no branch -- past a row's end they read the
+     * row's last arc again (or, for an empty row beside a non-empty one, an
+     * arc next to the row: the view then has arcs) and the distance of v
+     * itself -- so the block's arc loads, then its distance loads, are in
      * flight together */
-    constexpr int SU = T::SWEEP_U, SB = 4;
-    for (uint32_t v0 = r0; v0 < V.n; v0 += SU * R) {
-        uint32_t dv[SU], e0[SU], e1[SU], f0[SU], f1[SU], cin[SU], mask[SU];
-#pragma unroll
-        for (int k = 0; k < SU; ++k) {
-            const uint32_t v = v0 + k * R;
-            const bool in = v < V.n;
-            dv[k] = in ? dist_of<SC>(S, v) : DINF;
-            e0[k] = in ? V.rp[v] : 0u;
-            e1[k] = in ? V.rp[v + 1] : 0u;
-            f0[k] = in ? V.rrp[v] : 0u;
-            f1[k] = in ? V.rrp[v + 1] : 0u;
-            cin[k] = 0;
-            mask[k] = 0;
+    static_assert(T::SWEEP_U == 1, "the sweep takes one vertex per thread and step");
+    constexpr int SB = 4;
+    for (uint32_t v = r0; v < V.n; v += R) {
+        const uint32_t dv = dist_of<SC>(S, v);
+        const uint32_t e0 = V.rp[v], f0 = V.rrp[v];
+        uint32_t e1 = V.rp[v + 1], f1 = V.rrp[v + 1];
+        uint32_t cin = 0, mask = 0;
+        if (dv == DINF) {
+            e1 = e0;   /* unreached: nothing to test */
+            f1 = f0;
+        } else {
+            te += e1 - e0;
+            if (v == src) f1 = f0;   /* the source has no predecessors */
         }
+        const uint32_t osafe = e1 > e0 ? e1 - 1 : e0 > 0 ? e0 - 1 : 0u;
+        const uint32_t isafe = f1 > f0 ? f1 - 1 : f0 > 0 ? f0 - 1 : 0u;
+        for (uint32_t b = 0; e0 + b < e1 || f0 + b < f1; b += SB) {
+            uint2 ao[SB], ai[SB];
+            bool oo[SB], oi[SB];
 #pragma unroll
-        for (int k = 0; k < SU; ++k) {
-            if (dv[k] == DINF) {
-                e1[k] = e0[k];   /* unreached: nothing to test */
-                f1[k] = f0[k];
+            for (int jj = 0; jj < SB; ++jj) {
+                oo[jj] = e0 + b + jj < e1;
+                oi[jj] = f0 + b + jj < f1;
+                ao[jj] = V.arcs[oo[jj] ? e0 + b + jj : osafe];
+                ai[jj] = V.rarcs[oi[jj] ? f0 + b + jj : isafe];
+            }
+            uint32_t dout[SB], din[SB];
+#pragma unroll
+            for (int jj = 0; jj < SB; ++jj) {
+                dout[jj] = dist_of<SC>(S, oo[jj] ? ao[jj].x : v);
+                din[jj] = dist_of<SC>(S, oi[jj] ? ai[jj].x : v);
+            }
+#pragma unroll
+            for (int jj = 0; jj < SB; ++jj) {
+                if (oo[jj] && dout[jj] != DINF && (uint64_t)dv + ao[jj].y == dout[jj] && b + jj < 32)
+                    mask |= 1u << (b + jj);
+                cin += oi[jj] && din[jj] != DINF && (uint64_t)din[jj] + ai[jj].y == dv;
+            }
+        }
+        ulonglong2 pair = make_ulonglong2(0, 0);   /* unreached: sigma 0 */
+        if (dv != DINF) {
+            const uint32_t mk = e1 - e0 > MASK_ARCS ? MASK_NONE : mask;
+            if (PACKED) {
+                if (cin > PK_CMAX) ast32<SC>(&c.wide, 1u);
+                pair.x = v == src ? 1u : (uint64_t)cin << 48;
+                pair.y = ((uint64_t)e0 << 32) | mk;   /* the row start rides along */
             } else {
-                te += e1[k] - e0[k];
-                if (v0 + k * R == src) f1[k] = f0[k];   /* the source has no predecessors */
+                pair.x = v == src ? 1u : 0u;
+                pair.y = ((uint64_t)cin << 32) | mk;
             }
         }
-        for (uint32_t b = 0;; b += SB) {
-            bool any = false;
-#pragma unroll
-            for (int k = 0; k < SU; ++k) any = any || e0[k] + b < e1[k] || f0[k] + b < f1[k];
-            if (!any) break;
-            uint2 ao[SU][SB], ai[SU][SB];
-#pragma unroll
-            for (int k = 0; k < SU; ++k)
-#pragma unroll
-                for (int j = 0; j < SB; ++j) {
-                    ao[k][j] = e0[k] + b + j < e1[k] ? V.arcs[e0[k] + b + j] : make_uint2(0u, 0u);
-                    ai[k][j] = f0[k] + b + j < f1[k] ? V.rarcs[f0[k] + b + j] : make_uint2(0u, 0u);
-                }
-            uint32_t dout[SU][SB], din[SU][SB];
-#pragma unroll
-            for (int k = 0; k < SU; ++k)
-#pragma unroll
-                for (int j = 0; j < SB; ++j) {
-                    dout[k][j] = e0[k] + b + j < e1[k] ? dist_of<SC>(S, ao[k][j].x) : DINF;
-                    din[k][j] = f0[k] + b + j < f1[k] ? dist_of<SC>(S, ai[k][j].x) : DINF;
-                }
-#pragma unroll
-            for (int k = 0; k < SU; ++k)
-#pragma unroll
-                for (int j = 0; j < SB; ++j) {
-                    if (dout[k][j] != DINF && (uint64_t)dv[k] + ao[k][j].y == dout[k][j] && b + j < 32)
-                        mask[k] |= 1u << (b + j);
-                    cin[k] += din[k][j] != DINF && (uint64_t)din[k][j] + ai[k][j].y == dv[k];
-                }
-        }
-#pragma unroll
-        for (int k = 0; k < SU; ++k) {
-            const uint32_t v = v0 + k * R;
-            if (v >= V.n) break;
-            ulonglong2 pair = make_ulonglong2(0, 0);   /* unreached: sigma 0 */
-            if (dv[k] != DINF) {
-                const uint32_t mk = e1[k] - e0[k] > MASK_ARCS ? MASK_NONE : mask[k];
-                if (PACKED) {
-                    if (cin[k] > PK_CMAX) ast32<SC>(&c.wide, 1u);
-                    pair.x = v == src ? 1u : (uint64_t)cin[k] << 48;
-                    pair.y = ((uint64_t)e0[k] << 32) | mk;   /* the row start rides along */
-                } else {
-                    pair.x = v == src ? 1u : 0u;
-                    pair.y = ((uint64_t)cin[k] << 32) | mk;
-                }
-            }
-            reinterpret_cast<ulonglong2 *>(S.sd)[v] = pair;
-        }
+        reinterpret_cast<ulonglong2 *>(S.sd)[v] = pair;
     }
     Local &L = *t.loc;
     if (te && count_te) atomicAdd(&L.te, te);
@@ -1097,7 +1095,7 @@ struct Acc<1> {
  * final here, so plain loads see them after the level barrier.  With a
  * tight-arc mask the successors are known: d(w) = d(v) + w(v, w) and only
  * their (sigma, delta) pairs are read. */
-template <int PREC, bool ORDERED, int RB, int SUCC_REG>
+template <int PREC, bool ORDERED, int RB, int SUCC_REG, int MB>
 __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S, uint32_t v, bool packed)
 {
     Acc<PREC> acc;
@@ -1109,25 +1107,25 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
      * wide form's high half is its spent predecessor count) */
     const uint32_t e0 = packed && mask != MASK_NONE ? (uint32_t)(own.y >> 32) : V.rp[v];
     if (!ORDERED && mask != MASK_NONE) {
-        /* ascending arc order, RB successors' loads in flight together */
+        /* ascending arc order, MB successors' loads in flight together */
         uint32_t m = mask;
         while (m) {
-            uint32_t j[RB];
-            bool ok[RB];
+            uint32_t j[MB];
+            bool ok[MB];
 #pragma unroll
-            for (int k = 0; k < RB; ++k) {
+            for (int k = 0; k < MB; ++k) {
                 ok[k] = m != 0;
                 j[k] = ok[k] ? (uint32_t)__builtin_ctz(m) : 0u;
                 m &= m - 1;
             }
-            uint32_t w[RB];
+            uint32_t w[MB];
 #pragma unroll
-            for (int k = 0; k < RB; ++k) w[k] = ok[k] ? V.arcs[e0 + j[k]].x : 0u;
-            ulonglong2 sw[RB];
+            for (int k = 0; k < MB; ++k) w[k] = V.arcs[e0 + j[k]].x;   /* j[k] = 0 when not ok: arc e0 exists */
+            ulonglong2 sw[MB];
 #pragma unroll
-            for (int k = 0; k < RB; ++k) sw[k] = ok[k] ? sd_pair(S, w[k]) : make_ulonglong2(0, 0);
+            for (int k = 0; k < MB; ++k) sw[k] = sd_pair(S, w[k]);   /* loads without branches: in flight together */
 #pragma unroll
-            for (int k = 0; k < RB; ++k)
+            for (int k = 0; k < MB; ++k)
                 if (ok[k]) acc.add(sv_int, sv, sw[k].x, sw[k].y);
         }
         return acc;
@@ -1137,23 +1135,30 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         if (__popc(mask) <= SUCC_REG) {
             /* the successors' (sigma, delta) loads go out together; only the
              * sums follow the order */
+            if (mask == 0) return acc;   /* no DAG successor */
             const uint4 row = lds_row(V, v);
             const bool lr = row.x != ELL_WIDE;
-            uint64_t key[SUCC_REG];
-            uint32_t m = mask;
+            /* slots past the successors repeat the first one (a valid arc and
+             * vertex), so the arc loads, then the (sigma, delta) loads, carry
+             * no branch and are in flight together */
+            const uint32_t j0 = (uint32_t)__builtin_ctz(mask);
+            uint32_t m = mask, aj[SUCC_REG];
+            bool ok[SUCC_REG];
 #pragma unroll
             for (int j = 0; j < SUCC_REG; ++j) {
-                key[j] = 0;
-                if (m) {
-                    const uint32_t aj = (uint32_t)__builtin_ctz(m);
-                    const uint2 a = lr ? row_arc(row, v, aj) : V.arcs[e0 + aj];
-                    key[j] = ((uint64_t)(dv + a.y) << 32) | a.x;
-                    m &= m - 1;
-                }
+                ok[j] = m != 0;
+                aj[j] = ok[j] ? (uint32_t)__builtin_ctz(m) : j0;
+                m &= m - 1;
             }
+            uint2 a[SUCC_REG];
+#pragma unroll
+            for (int j = 0; j < SUCC_REG; ++j) a[j] = lr ? row_arc(row, v, aj[j]) : V.arcs[e0 + aj[j]];
+            uint64_t key[SUCC_REG];
+#pragma unroll
+            for (int j = 0; j < SUCC_REG; ++j) key[j] = ok[j] ? ((uint64_t)(dv + a[j].y) << 32) | a[j].x : 0;
             ulonglong2 sw[SUCC_REG];
 #pragma unroll
-            for (int j = 0; j < SUCC_REG; ++j) sw[j] = key[j] ? sd_pair(S, (uint32_t)key[j]) : make_ulonglong2(0, 0);
+            for (int j = 0; j < SUCC_REG; ++j) sw[j] = sd_pair(S, a[j].x);
 #pragma unroll
             for (int r = 0; r < SUCC_REG; ++r) {
                 uint64_t best = 0;
@@ -1195,32 +1200,38 @@ __device__ __forceinline__ Acc<PREC> fold_successors(const View &V, const Slot S
         for (uint32_t eb = e0; eb < e1; eb += RB) {
             uint2 a[RB];
 #pragma unroll
-            for (int j = 0; j < RB; ++j) a[j] = eb + j < e1 ? V.arcs[eb + j] : make_uint2(0u, 0u);
+            /* past the row's end: arc eb again (valid), so the arc, distance
+             * and (sigma, delta) loads carry no branch */
+            for (int j = 0; j < RB; ++j) a[j] = V.arcs[eb + j < e1 ? eb + j : eb];
+            uint32_t dw[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j) dw[j] = dist_final(S, a[j].x);
             bool tight[RB];
 #pragma unroll
-            for (int j = 0; j < RB; ++j) tight[j] = eb + j < e1 && (uint64_t)dv + a[j].y == dist_final(S, a[j].x);
+            for (int j = 0; j < RB; ++j) tight[j] = eb + j < e1 && (uint64_t)dv + a[j].y == dw[j];
             ulonglong2 sw[RB];
 #pragma unroll
-            for (int j = 0; j < RB; ++j) sw[j] = tight[j] ? sd_pair(S, a[j].x) : make_ulonglong2(0, 0);
+            for (int j = 0; j < RB; ++j) sw[j] = sd_pair(S, a[j].x);
 #pragma unroll
             for (int j = 0; j < RB; ++j)
                 if (tight[j]) acc.add(sv_int, sv, sw[j].x, sw[j].y);
         }
         return acc;
     }
+    if (e1 == e0) return acc;   /* no out-arc */
     if (e1 - e0 <= (uint32_t)SUCC_REG) {
+        /* past the row's end: arc e0 again (valid), so the loads carry no
+         * branch */
         uint2 a[SUCC_REG];
 #pragma unroll
-        for (int j = 0; j < SUCC_REG; ++j) a[j] = e0 + j < e1 ? V.arcs[e0 + j] : make_uint2(0u, 0u);
+        for (int j = 0; j < SUCC_REG; ++j) a[j] = V.arcs[e0 + j < e1 ? e0 + j : e0];
+        uint32_t dwj[SUCC_REG];
+#pragma unroll
+        for (int j = 0; j < SUCC_REG; ++j) dwj[j] = dist_final(S, a[j].x);
         uint64_t key[SUCC_REG];
 #pragma unroll
-        for (int j = 0; j < SUCC_REG; ++j) {
-            key[j] = 0;
-            if (e0 + j < e1) {
-                const uint32_t dw = dist_final(S, a[j].x);
-                if ((uint64_t)dv + a[j].y == dw) key[j] = ((uint64_t)dw << 32) | a[j].x;
-            }
-        }
+        for (int j = 0; j < SUCC_REG; ++j)
+            key[j] = e0 + j < e1 && (uint64_t)dv + a[j].y == dwj[j] ? ((uint64_t)dwj[j] << 32) | a[j].x : 0;
         /* keys are distinct and nonzero for successors (dw >= 1); the low
          * half of the best key is the successor's id */
 #pragma unroll
@@ -1274,7 +1285,7 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
         for (uint32_t i = i0 + r0; i < i1; i += R) {
             const uint32_t v = S.qa[i];
             if (v == src) continue;
-            const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC>(V, S, v, packed);
+            const Acc<PREC> acc = fold_successors<PREC, ORDERED, T::RB, T::SUCC, T::MB>(V, S, v, packed);
             S.sd[2 * (uint64_t)v + 1] = acc.bits();
         }
         t.sync();
@@ -1290,7 +1301,7 @@ __device__ __forceinline__ void phase_dependencies(const BrandesParams &p, const
 #pragma unroll
         for (int k = 0; k < OU; ++k) {
             const uint32_t v = v0 + k * R;
-            b[k] = v < V.n ? S.sd[2 * (uint64_t)v + 1] : 0;
+            b[k] = S.sd[2 * (uint64_t)(v < V.n ? v : V.n - 1) + 1];   /* unconditional: in flight together */
         }
 #pragma unroll
         for (int k = 0; k < OU; ++k) {
