@@ -367,3 +367,35 @@ def test_ballot_append_same_results(gpu_ctx):
     assert np.array_equal(agg[0], base[0]) and np.array_equal(agg[2], base[2])
     assert all(np.array_equal(a, b) for a, b in zip(agg[1], base[1]))
     assert np.array_equal(plain, og.single_source_dep(int(srcs[0])))
+
+
+def test_wide_weight_rows_bitexact(gpu_ctx):
+    """Weights beyond a short row's 16 bits (0xFFFE) mark every row wide, so
+    the distance phase relaxes each short row as empty and takes the CSR
+    path, and the LDS view holds only wide rows: distances, sigma and FP64
+    dependencies stay bit-exact against the oracle, and the cluster
+    evaluation with the rows in LDS equals the HBM-row kernel."""
+    n, (s, d, w) = F.generate_config(0)
+    rng = np.random.default_rng(7)
+    w = (w.astype(np.uint64) * 1000 + rng.integers(0, 1000, len(w))).astype(np.uint32)   # 1,000 .. 100,999
+    assert w.max() > 0xFFFE
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    srcs = rng.choice(n, 24, replace=False)
+    check_sssp(g, og, srcs)
+    dep = g.dep_batch(srcs)
+    for i, x in enumerate(srcs):
+        assert np.array_equal(dep[i], og.single_source_dep(int(x))), f"source {x}"
+    comm, _ = og.louvain([1, 2, 3, 4], 0.01)
+    cl = g.clusters(comm_of=comm)
+    gpu_ctx.set_option("lds_view", 0)
+    try:
+        bc0, L0, C0 = cl.evaluate()
+        gpu_ctx.set_option("lds_view", 1)
+        bc1, L1, C1 = cl.evaluate()
+    finally:
+        gpu_ctx.set_option("lds_view", 2)
+    assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
+    c0 = 5
+    verts = cl.members(c0)
+    obc, ob, oL, oC = og.cluster_eval(verts)
+    assert np.array_equal(bc0[verts], obc[verts])
