@@ -379,7 +379,7 @@ __device__ __forceinline__ void wave_sync()
 template <uint32_t LQN>
 struct TeamWaveT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1, MB = 4;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 2, MB = 4;
     static constexpr uint32_t LQ = LQN, LANES = 64;
     Ctl *ctl;
     Local *loc;
@@ -403,7 +403,7 @@ struct TeamWaveT {
 template <uint32_t LQN>
 struct TeamHalfT {
     static constexpr int SC = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 1, MB = 4;
+    static constexpr int RB = 4, SUCC = 4, SWEEP_U = 2, MB = 4;
     static constexpr uint32_t LQ = LQN, LANES = 32;
     Ctl *ctl;
     Local *loc;
@@ -886,64 +886,102 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
     const uint32_t r0 = t.rank(), R = t.size();
     Ctl &c = *t.ctl;
     unsigned long long te = 0;
-    /* one vertex per thread and step, its out- and in-arcs SB at a time.
+    /* SU vertices per thread and step, their out- and in-arcs SB at a time.
      * The loads of a block carry no branch -- past a row's end they read the
      * row's last arc again (or, for an empty row beside a non-empty one, an
-     * arc next to the row: the view then has arcs) and the distance of v
-     * itself -- so the block's arc loads, then its distance loads, are in
+     * arc next to the row: the view then has arcs), a step's vertex past the
+     * view's end is its last vertex again, and a slot's distance load reads
+     * v's own -- so the step's arc loads, then its distance loads, are in
      * flight together */
-    static_assert(T::SWEEP_U == 1, "the sweep takes one vertex per thread and step");
-    constexpr int SB = 4;
-    for (uint32_t v = r0; v < V.n; v += R) {
-        const uint32_t dv = dist_of<SC>(S, v);
-        const uint32_t e0 = V.rp[v], f0 = V.rrp[v];
-        uint32_t e1 = V.rp[v + 1], f1 = V.rrp[v + 1];
-        uint32_t cin = 0, mask = 0;
-        if (dv == DINF) {
-            e1 = e0;   /* unreached: nothing to test */
-            f1 = f0;
-        } else {
-            te += e1 - e0;
-            if (v == src) f1 = f0;   /* the source has no predecessors */
+    constexpr int SU = T::SWEEP_U, SB = 4;
+    for (uint32_t v0 = r0; v0 < V.n; v0 += SU * R) {
+        uint32_t vv[SU], dv[SU], e0[SU], e1[SU], f0[SU], f1[SU], os[SU], is[SU], cin[SU], mask[SU];
+        bool in[SU];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            in[k] = v0 + k * R < V.n;
+            vv[k] = in[k] ? v0 + k * R : V.n - 1;
+            dv[k] = dist_of<SC>(S, vv[k]);
+            e0[k] = V.rp[vv[k]];
+            e1[k] = V.rp[vv[k] + 1];
+            f0[k] = V.rrp[vv[k]];
+            f1[k] = V.rrp[vv[k] + 1];
         }
-        const uint32_t osafe = e1 > e0 ? e1 - 1 : e0 > 0 ? e0 - 1 : 0u;
-        const uint32_t isafe = f1 > f0 ? f1 - 1 : f0 > 0 ? f0 - 1 : 0u;
-        for (uint32_t b = 0; e0 + b < e1 || f0 + b < f1; b += SB) {
-            uint2 ao[SB], ai[SB];
-            bool oo[SB], oi[SB];
 #pragma unroll
-            for (int jj = 0; jj < SB; ++jj) {
-                oo[jj] = e0 + b + jj < e1;
-                oi[jj] = f0 + b + jj < f1;
-                ao[jj] = V.arcs[oo[jj] ? e0 + b + jj : osafe];
-                ai[jj] = V.rarcs[oi[jj] ? f0 + b + jj : isafe];
-            }
-            uint32_t dout[SB], din[SB];
-#pragma unroll
-            for (int jj = 0; jj < SB; ++jj) {
-                dout[jj] = dist_of<SC>(S, oo[jj] ? ao[jj].x : v);
-                din[jj] = dist_of<SC>(S, oi[jj] ? ai[jj].x : v);
-            }
-#pragma unroll
-            for (int jj = 0; jj < SB; ++jj) {
-                if (oo[jj] && dout[jj] != DINF && (uint64_t)dv + ao[jj].y == dout[jj] && b + jj < 32)
-                    mask |= 1u << (b + jj);
-                cin += oi[jj] && din[jj] != DINF && (uint64_t)din[jj] + ai[jj].y == dv;
-            }
-        }
-        ulonglong2 pair = make_ulonglong2(0, 0);   /* unreached: sigma 0 */
-        if (dv != DINF) {
-            const uint32_t mk = e1 - e0 > MASK_ARCS ? MASK_NONE : mask;
-            if (PACKED) {
-                if (cin > PK_CMAX) ast32<SC>(&c.wide, 1u);
-                pair.x = v == src ? 1u : (uint64_t)cin << 48;
-                pair.y = ((uint64_t)e0 << 32) | mk;   /* the row start rides along */
+        for (int k = 0; k < SU; ++k) {
+            cin[k] = 0;
+            mask[k] = 0;
+            if (!in[k] || dv[k] == DINF) {
+                e1[k] = e0[k];   /* unreached (or no vertex): nothing to test */
+                f1[k] = f0[k];
             } else {
-                pair.x = v == src ? 1u : 0u;
-                pair.y = ((uint64_t)cin << 32) | mk;
+                te += e1[k] - e0[k];
+                if (vv[k] == src) f1[k] = f0[k];   /* the source has no predecessors */
             }
+            os[k] = e1[k] > e0[k] ? e1[k] - 1 : e0[k] > 0 ? e0[k] - 1 : 0u;
+            is[k] = f1[k] > f0[k] ? f1[k] - 1 : f0[k] > 0 ? f0[k] - 1 : 0u;
         }
-        reinterpret_cast<ulonglong2 *>(S.sd)[v] = pair;
+        for (uint32_t b = 0;; b += SB) {
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < SU; ++k) any = any || e0[k] + b < e1[k] || f0[k] + b < f1[k];
+            if (!any) break;
+            /* a vertex with no arc left in this block reads arc os / is, which
+             * exists when any row of the step is non-empty only if its own
+             * view has arcs: clamp it to a step vertex's safe arc that does */
+            uint32_t osafe = 0, isafe = 0;
+#pragma unroll
+            for (int k = SU - 1; k >= 0; --k) {
+                if (e1[k] > e0[k] || f1[k] > f0[k]) {
+                    osafe = os[k];
+                    isafe = is[k];
+                }
+            }
+            uint2 ao[SU][SB], ai[SU][SB];
+            bool oo[SU][SB], oi[SU][SB];
+#pragma unroll
+            for (int k = 0; k < SU; ++k)
+#pragma unroll
+                for (int jj = 0; jj < SB; ++jj) {
+                    oo[k][jj] = e0[k] + b + jj < e1[k];
+                    oi[k][jj] = f0[k] + b + jj < f1[k];
+                    ao[k][jj] = V.arcs[oo[k][jj] ? e0[k] + b + jj : osafe];
+                    ai[k][jj] = V.rarcs[oi[k][jj] ? f0[k] + b + jj : isafe];
+                }
+            uint32_t dout[SU][SB], din[SU][SB];
+#pragma unroll
+            for (int k = 0; k < SU; ++k)
+#pragma unroll
+                for (int jj = 0; jj < SB; ++jj) {
+                    dout[k][jj] = dist_of<SC>(S, oo[k][jj] ? ao[k][jj].x : vv[k]);
+                    din[k][jj] = dist_of<SC>(S, oi[k][jj] ? ai[k][jj].x : vv[k]);
+                }
+#pragma unroll
+            for (int k = 0; k < SU; ++k)
+#pragma unroll
+                for (int jj = 0; jj < SB; ++jj) {
+                    if (oo[k][jj] && dout[k][jj] != DINF && (uint64_t)dv[k] + ao[k][jj].y == dout[k][jj] && b + jj < 32)
+                        mask[k] |= 1u << (b + jj);
+                    cin[k] += oi[k][jj] && din[k][jj] != DINF && (uint64_t)din[k][jj] + ai[k][jj].y == dv[k];
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            if (!in[k]) break;
+            ulonglong2 pair = make_ulonglong2(0, 0);   /* unreached: sigma 0 */
+            if (dv[k] != DINF) {
+                const uint32_t mk = e1[k] - e0[k] > MASK_ARCS ? MASK_NONE : mask[k];
+                if (PACKED) {
+                    if (cin[k] > PK_CMAX) ast32<SC>(&c.wide, 1u);
+                    pair.x = vv[k] == src ? 1u : (uint64_t)cin[k] << 48;
+                    pair.y = ((uint64_t)e0[k] << 32) | mk;   /* the row start rides along */
+                } else {
+                    pair.x = vv[k] == src ? 1u : 0u;
+                    pair.y = ((uint64_t)cin[k] << 32) | mk;
+                }
+            }
+            reinterpret_cast<ulonglong2 *>(S.sd)[vv[k]] = pair;
+        }
     }
     Local &L = *t.loc;
     if (te && count_te) atomicAdd(&L.te, te);
