@@ -2,8 +2,9 @@
  * Driver for the reference's own headers -- TEST INFRASTRUCTURE ONLY.
  *
  * Built by oracle/Makefile (target 'ref') against the headers under
- * /root/reference/libfastbc/include into oracle/_ref/ref_driver, in the
- * survey container only.  Only headers that compile with the image's own
+ * /root/reference/libfastbc/include into oracle/_ref/ref_driver, only where
+ * that tree is present; the built binaries travel with the repository tree
+ * (they are git-ignored).  Only headers that compile with the image's own
  * toolchain are included; SubGraph.h, ClusteredBrandesBC.h, the two pivot
  * selectors and LouvainGraphPartition.h need spdlog, which is not installed,
  * so those parts are restated below (marked) or checked by other means.
