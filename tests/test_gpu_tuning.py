@@ -112,3 +112,30 @@ def test_lds_view_cluster_eval_identical(gpu_ctx, cfg):
           f"{int(np.bincount(comm).max())} vertices", flush=True)
     assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
     assert np.array_equal(bc2, bc0) and np.array_equal(L2, L0) and np.array_equal(C2, C0)
+
+
+def test_lds_view_irregular_partition(gpu_ctx):
+    """A partition of cfg1's grid into row-major strips of 1 to ~5,000
+    vertices (singletons and pairs included, so views change size from task
+    to task and the staged view is replaced often): the LDS-view kernel,
+    forced on, equals the HBM-row kernel bit for bit."""
+    n, (s, d, w) = F.generate_config(1)
+    rng = np.random.default_rng(11)
+    sizes = []
+    while sum(sizes) < n:
+        sizes.append(int(rng.choice([1, 2, 3, 17, 250, 1000, 2047, 2049, 5000])))
+    comm = np.repeat(np.arange(len(sizes), dtype=np.int32), sizes)[:n]
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))
+    cl = g.clusters(comm_of=comm)
+    gpu_ctx.set_option("autotune", 0)
+    try:
+        gpu_ctx.set_option("lds_view", 0)
+        bc0, L0, C0 = cl.evaluate()
+        gpu_ctx.set_option("lds_view", 1)
+        bc1, L1, C1 = cl.evaluate()
+        assert gpu_ctx.last_counters()["lds_view"] == 1
+    finally:
+        gpu_ctx.set_option("lds_view", 2)
+        gpu_ctx.set_option("autotune", 1)
+    print(f"{len(sizes)} strips of 1..5000 vertices", flush=True)
+    assert np.array_equal(bc1, bc0) and np.array_equal(L1, L0) and np.array_equal(C1, C0)
