@@ -67,6 +67,7 @@ struct fbc_ctx {
     uint8_t *noalloc = nullptr;   /* this device's address of 'scratch' */
     uint32_t autotune = 1;        /* tune delta / team form on large batched runs */
     uint32_t ballot_append = 0;   /* wavefront-aggregated list appends */
+    uint32_t tune_log = 0;        /* autotuner: a line per trial launch on stderr */
     /* cluster evaluation on the LDS-view kernel: 0 never, 1 always, 2 when
      * the clusters whose rows fit in LDS carry at least half of the run's
      * work (lds_fit_share, set by the cluster evaluation for each launch) and
@@ -388,6 +389,11 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
         use.lds = def.lds;   /* a property of this run's views, not of the graph */
         ctx->last_tuned = 2;
     } else if (tunable) {
+        auto tlog = [&](const char *step, uint32_t delta, uint32_t half, uint64_t n, double ms, double te) {
+            if (ctx->tune_log)
+                std::fprintf(stderr, "[fbc tune] mode %d %s: delta %u, half_team_max %u, %llu tasks, %.1f ms, %.4g edges/ms\n",
+                             mode, step, delta, half, (unsigned long long)n, ms, te / std::max(ms, 1e-3));
+        };
         uint32_t slots = 0;
         int rc = ensure_scratch(ctx, p.nmax, p.ntasks, &slots, def.half_max, def.lds);
         if (rc) return rc;
@@ -409,8 +415,10 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
                 q.trial_stats = ctx->d_trial;
                 HIP_TRY(ctx, hipMemsetAsync(ctx->d_trial, 0, 2 * fbcdev::TRIAL_MAX * sizeof(unsigned long long),
                                             ctx->stream));
-                if ((rc = timed_launch(ctx, mode, prec, ordered, q, err, nullptr, nullptr))) return rc;
+                double ms = 0, te = 0;
+                if ((rc = timed_launch(ctx, mode, prec, ordered, q, err, &ms, &te))) return rc;
                 done += slots;
+                tlog("region", 0, def.half_max, slots, ms, te);
             }
             unsigned long long st[2 * fbcdev::TRIAL_MAX] = {};
             HIP_TRY(ctx, hipMemcpy(st, ctx->d_trial, sizeof(st), hipMemcpyDeviceToHost));
@@ -418,6 +426,9 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             double br = -1;
             for (uint32_t k = 0; k < C; ++k) {
                 const double rate = st[2 * k + 1] ? (double)st[2 * k] / (double)st[2 * k + 1] : 0.0;
+                if (ctx->tune_log)
+                    std::fprintf(stderr, "[fbc tune]   candidate delta %u: %llu edges in %llu busy ticks, %.4g per tick\n",
+                                 cand[k], st[2 * k], st[2 * k + 1], rate);
                 if (rate > br) {
                     br = rate;
                     bi = k;
@@ -435,6 +446,7 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
                 if (e) return e;
                 done += n;
                 *rate = te / std::max(ms, 1e-3);
+                tlog("trial", u.delta, u.half_max, n, ms, te);
                 return 0;
             };
             auto room = [&](uint64_t n) { return p.ntasks - done >= n + slots; };
@@ -485,7 +497,11 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             r.task_base = (uint32_t)done;
             r.ntasks = p.ntasks - (uint32_t)done;
             if ((rc = apply_choice(ctx, r, use))) return rc;
-            return r.ntasks ? timed_launch(ctx, mode, prec, ordered, r, err, nullptr, nullptr) : 0;
+            if (!r.ntasks) return 0;
+            double ms = 0, te = 0;
+            if ((rc = timed_launch(ctx, mode, prec, ordered, r, err, &ms, &te))) return rc;
+            tlog("rest", use.delta, use.half_max, r.ntasks, ms, te);
+            return 0;
         }
     }
     ctx->last_delta = use.delta;
@@ -686,6 +702,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "autotune")) ctx->autotune = v ? 1 : 0;
     else if (!std::strcmp(name, "ballot_append")) ctx->ballot_append = v ? 1 : 0;
     else if (!std::strcmp(name, "lds_view")) ctx->lds_view = std::min<uint32_t>(v, 2);
+    else if (!std::strcmp(name, "tune_log")) ctx->tune_log = v ? 1 : 0;
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);

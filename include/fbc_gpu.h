@@ -97,8 +97,12 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * (cluster evaluation with each cluster's short rows staged in LDS, shared by
  * the 16 wavefront -- or 32 half-wavefront -- teams of a 1024-thread
  * workgroup, one workgroup per CU; clusters too large for the LDS read their
- * rows from HBM: 0 never, 1 always, 2, the default: when the autotuner
- * measures it faster on the graph).  No counterpart in the reference. */
+ * rows from HBM: 0 never, 1 always, 2, the default: when the clusters whose
+ * rows fit carry at least half of the run's work, n_c x m_c summed, and every
+ * team gets 32 tasks or more), "tune_log" (1: the autotuner prints a line per
+ * trial launch on stderr -- its delta, tasks, time and edges per ms, and each
+ * region candidate's edges per busy tick; 0, the default: silent).  No
+ * counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].

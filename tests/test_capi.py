@@ -32,3 +32,16 @@ def test_null_arguments_rejected_without_device():
     assert lib.fbc_set_tuning(None, 0, 0) == -1
     assert lib.fbc_last_stats(None, None, None) == -1
     assert lib.fbc_last_error(None) == b"no context"
+
+
+def test_option_names_documented():
+    """Every name fbc_set_option accepts is documented in include/fbc_gpu.h
+    and in the Python wrapper's set_option."""
+    src = open(os.path.join(ROOT, "fast-bc_amd", "csrc", "fbc_capi.cpp")).read()
+    names = set(re.findall(r'strcmp\(name, "([a-z_0-9]+)"\)', src))
+    assert {"delta", "autotune", "lds_view", "tune_log"} <= names
+    header = open(os.path.join(ROOT, "include", "fbc_gpu.h")).read()
+    doc = F.Context.set_option.__doc__
+    for n in sorted(names):
+        assert f'"{n}"' in header, n
+        assert f'"{n}"' in doc, n

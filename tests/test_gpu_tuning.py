@@ -44,6 +44,30 @@ def test_autotune_exact_bc(gpu_ctx):
         assert (np.abs(bc - ref) / np.maximum(np.abs(ref), floor)).max() <= 1e-9
 
 
+def test_tune_log_lines(gpu_ctx, capfd):
+    """tune_log prints the trial launches on stderr and changes no result."""
+    n, (s, d, w) = F.generate_config(0)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))   # a new graph: tuned afresh
+    ref = O.Graph.from_arcs(s, d, w.astype(np.float64)).exact_bc(THREADS)
+    floor = 1e-12 * np.abs(ref).max()
+    gpu_ctx.set_tuning(0, 64)
+    gpu_ctx.set_option("tune_log", 1)
+    try:
+        bc = g.exact_bc()
+        c = gpu_ctx.last_counters()
+    finally:
+        gpu_ctx.set_option("tune_log", 0)
+        gpu_ctx.set_tuning(0, 0)
+    err = capfd.readouterr().err
+    lines = [x for x in err.splitlines() if x.startswith("[fbc tune]")]
+    print("\n".join(lines), flush=True)
+    assert c["tuned"] == 1
+    assert any(" region: " in x for x in lines)
+    assert sum("candidate delta" in x for x in lines) >= 2
+    assert any(f" rest: delta {c['delta']}," in x for x in lines)
+    assert (np.abs(bc - ref) / np.maximum(np.abs(ref), floor)).max() <= 1e-9
+
+
 def test_autotune_cluster_eval_identical(gpu_ctx):
     """Cluster evaluation tuned (half-wavefront or wavefront teams, three
     deltas) equals the fixed configuration bit for bit: the intra-cluster
