@@ -336,7 +336,7 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build measures a single "
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): it measures a single "
               f"MI355X; the pivot and (cluster, source) queues would be split across GPUs", file=sys.stderr)
         sys.exit(2)
 

@@ -10,7 +10,7 @@
  *     -t, --threads            host threads (Louvain instances, pivot selection)
  *     -o, --output             output file (default bc.txt; must not exist)
  *     -d, --debug              log level (trace|debug|info|warning|error|critical|off)
- *   options of this build:
+ *   options beyond the reference's:
  *         --device             HIP device (default 0)
  *         --precision          f64 | f32c: dependency arithmetic (default f64)
  *         --form               batched | plain: all pivots in one launch (default),
@@ -210,11 +210,11 @@ int main(int argc, char **argv)
         {'t', "threads", "Maximum number of threads used in parallel computation", true},
         {'o', "output", "Output file path", true},
         {'d', "debug", "Logger level (trace|debug|info|warning|error|critical|off)", true},
-        {0, "device", "HIP device index (this build)", true},
-        {0, "precision", "Dependency arithmetic: f64 | f32c (this build)", true},
-        {0, "form", "Pivot phase: batched | plain (this build)", true},
-        {0, "full-output", "Also write unclamped %.17g values to this file (this build)", true},
-        {0, "csr-cache", "Binary CSR cache of the input: read when it matches the input, else written (this build)",
+        {0, "device", "HIP device index (not in the reference)", true},
+        {0, "precision", "Dependency arithmetic: f64 | f32c (not in the reference)", true},
+        {0, "form", "Pivot phase: batched | plain (not in the reference)", true},
+        {0, "full-output", "Also write unclamped %.17g values to this file (not in the reference)", true},
+        {0, "csr-cache", "Binary CSR cache of the input: read when it matches the input, else written (not in the reference)",
          true},
     };
     std::string seedstr, outpath = "bc.txt", level = "info", fullout, precstr = "f64", formstr = "batched", csrcache;

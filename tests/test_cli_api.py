@@ -1,5 +1,6 @@
 """The drop-in surfaces: the reference-named C++ API (tests/cpp/test_api.cpp)
 and the fbc command line (grammar, exit codes, bc.txt format of main.cpp)."""
+import fcntl
 import os
 import subprocess
 
@@ -28,8 +29,14 @@ def cfg0_files(tmp_path_factory):
 def build_api():
     """make is incremental and the program depends on every host header and on
     the library, so a binary built against an older struct layout is rebuilt
-    rather than run."""
-    subprocess.run(["make", "-s", "-C", os.path.dirname(API)], check=True)
+    rather than run.  A file lock keeps parallel test workers from running the
+    build at the same time (one could otherwise run a half-written binary)."""
+    with open(os.path.join(os.path.dirname(API), ".build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        try:
+            subprocess.run(["make", "-s", "-C", os.path.dirname(API)], check=True)
+        finally:
+            fcntl.flock(lk, fcntl.LOCK_UN)
 
 
 def test_cpp_api_host_side(cfg0_files):
