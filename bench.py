@@ -358,12 +358,13 @@ def main():
         r["call_s"] = time.perf_counter() - t
         return r
 
-    cold = None
+    cold = cold_phases = None
     for i in range(args.warmup):
         r = step(F.PREC_F32C)
         if cold is None:
             cold = r["phases"]["total_s"]
-        log(f"warmup {i}: computeBC {r['phases']['total_s']:.2f}s tuning {r['tuning']}")
+            cold_phases = {k: round(v, 3) for k, v in r["phases"].items() if k.endswith("_s")}
+        log(f"warmup {i}: computeBC {r['phases']['total_s']:.2f}s phases {r['phases']} tuning {r['tuning']}")
     runs = []
     for i in range(args.steps):
         r = step(F.PREC_F32C)
@@ -402,6 +403,7 @@ def main():
         # the process's first computeBC on the graph: device upload, the
         # relabelled copy, slot memory and the autotuner's trial launches
         "cold_first_call_s": round(cold, 3) if cold is not None else None,
+        "cold_first_call_phases_s": cold_phases,
         "tuning": last["tuning"],
         "phases_s": {k: round(v, 3) for k, v in ph.items() if k.endswith("_s")},
         "roofline": {
