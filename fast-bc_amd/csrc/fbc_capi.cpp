@@ -357,6 +357,11 @@ int tune_slot(int mode)
  *    1197 / 1357 ms at delta 511 / 8 / 12 / 32), but a greedier candidate
  *    takes memory bandwidth from the others in a mixed launch, so close ones
  *    are ranked by:
+ *    The pivot phase of a clustered run skips this launch when the cluster
+ *    evaluation has tuned on the same graph (its views are pieces of it, and
+ *    its region launch is short): the walk starts at that choice instead.
+ *    A whole-graph region launch pays for its slowest candidates: on the
+ *    1M-vertex grid 850 ms for work that the chosen delta does in ~460.
  * 2. Walk: launches of one task per slot, all at one delta, by edges per
  *    millisecond: the region's choice, then its smaller neighbours while
  *    they win, else its larger ones while they win (the 1M-vertex grid's
@@ -405,33 +410,51 @@ int run_batched(fbc_ctx *ctx, fbc_graph *g, int mode, int prec, bool ordered, Br
             }
             const uint32_t C = (uint32_t)cand.size();
             uint64_t done = 0;
-            /* 1. region */
-            {
-                BrandesParams q = p;
-                q.ntasks = slots;
-                if ((rc = apply_choice(ctx, q, def))) return rc;
-                q.trial_sets = C;
-                for (uint32_t k = 0; k < C; ++k) q.trial_delta[k] = cand[k];
-                q.trial_stats = ctx->d_trial;
-                HIP_TRY(ctx, hipMemsetAsync(ctx->d_trial, 0, 2 * fbcdev::TRIAL_MAX * sizeof(unsigned long long),
-                                            ctx->stream));
-                double ms = 0, te = 0;
-                if ((rc = timed_launch(ctx, mode, prec, ordered, q, err, &ms, &te))) return rc;
-                done += slots;
-                tlog("region", 0, def.half_max, slots, ms, te);
-            }
-            unsigned long long st[2 * fbcdev::TRIAL_MAX] = {};
-            HIP_TRY(ctx, hipMemcpy(st, ctx->d_trial, sizeof(st), hipMemcpyDeviceToHost));
             uint32_t bi = 0;
-            double br = -1;
-            for (uint32_t k = 0; k < C; ++k) {
-                const double rate = st[2 * k + 1] ? (double)st[2 * k] / (double)st[2 * k + 1] : 0.0;
+            const TuneChoice &cl = g->tuned[tune_slot(fbcdev::MODE_CLUSTER)];
+            if (mode == fbcdev::MODE_PIVOT && cl.set) {
+                /* the cluster evaluation's choice on this graph: the walk
+                 * starts at the candidate nearest to it */
+                double bd = 1e300;
+                for (uint32_t k = 0; k < C; ++k) {
+                    const double dd = std::fabs(std::log((double)cand[k] / (double)std::max(cl.delta, 1u)));
+                    if (dd < bd) {
+                        bd = dd;
+                        bi = k;
+                    }
+                }
                 if (ctx->tune_log)
-                    std::fprintf(stderr, "[fbc tune]   candidate delta %u: %llu edges in %llu busy ticks, %.4g per tick\n",
-                                 cand[k], st[2 * k], st[2 * k + 1], rate);
-                if (rate > br) {
-                    br = rate;
-                    bi = k;
+                    std::fprintf(stderr, "[fbc tune] mode %d: no region launch, the walk starts at delta %u (cluster "
+                                         "evaluation's choice %u)\n",
+                                 mode, cand[bi], cl.delta);
+            } else {
+                /* 1. region */
+                {
+                    BrandesParams q = p;
+                    q.ntasks = slots;
+                    if ((rc = apply_choice(ctx, q, def))) return rc;
+                    q.trial_sets = C;
+                    for (uint32_t k = 0; k < C; ++k) q.trial_delta[k] = cand[k];
+                    q.trial_stats = ctx->d_trial;
+                    HIP_TRY(ctx, hipMemsetAsync(ctx->d_trial, 0, 2 * fbcdev::TRIAL_MAX * sizeof(unsigned long long),
+                                                ctx->stream));
+                    double ms = 0, te = 0;
+                    if ((rc = timed_launch(ctx, mode, prec, ordered, q, err, &ms, &te))) return rc;
+                    done += slots;
+                    tlog("region", 0, def.half_max, slots, ms, te);
+                }
+                unsigned long long st[2 * fbcdev::TRIAL_MAX] = {};
+                HIP_TRY(ctx, hipMemcpy(st, ctx->d_trial, sizeof(st), hipMemcpyDeviceToHost));
+                double br = -1;
+                for (uint32_t k = 0; k < C; ++k) {
+                    const double rate = st[2 * k + 1] ? (double)st[2 * k] / (double)st[2 * k + 1] : 0.0;
+                    if (ctx->tune_log)
+                        std::fprintf(stderr, "[fbc tune]   candidate delta %u: %llu edges in %llu busy ticks, %.4g per tick\n",
+                                     cand[k], st[2 * k], st[2 * k + 1], rate);
+                    if (rate > br) {
+                        br = rate;
+                        bi = k;
+                    }
                 }
             }
             /* one launch of n tasks at one configuration; its edges per ms */

@@ -85,9 +85,11 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * phase, exact BC or the cluster evaluation on a graph with no tuned
  * configuration yet tunes on its first tasks -- one launch of one task per
  * slot with mixed delta candidates (powers of two of the mean arc weight, 8
- * down to 1/16) locates the region, launches at one delta each walk to the
- * best neighbour, and for views small enough for half-wavefront teams one
- * launch of whole wavefronts decides the team form -- and keeps the choice
+ * down to 1/16) locates the region (the pivot phase skips it when the
+ * cluster evaluation has tuned on the same graph, and walks from that
+ * choice), launches at one delta each walk to the best neighbour, and for
+ * views small enough for half-wavefront teams one launch of whole
+ * wavefronts decides the team form -- and keeps the choice
  * for the rest of the run and for later runs on the graph; no task runs
  * twice; an explicit "delta" turns it off; 0: off), "ballot_append" (1:
  * queue and level appends take one

@@ -68,6 +68,41 @@ def test_tune_log_lines(gpu_ctx, capfd):
     assert (np.abs(bc - ref) / np.maximum(np.abs(ref), floor)).max() <= 1e-9
 
 
+def test_pivot_walk_starts_from_cluster_choice(gpu_ctx, capfd):
+    """After the cluster evaluation has tuned on a graph, the pivot phase
+    tunes without a region launch, walking from the cluster's delta, and its
+    sums equal an untuned run's."""
+    n, (s, d, w) = F.generate_config(0)
+    g = F.Graph(gpu_ctx, n, *F.csr_from_arcs(n, s, d, w))   # a new graph: nothing tuned yet
+    comm, k = F.louvain(s, d, w, [1, 2, 3, 4], 0.01)
+    cl = g.clusters(comm_of=comm)
+    piv = np.arange(0, n, 2, dtype=np.uint32)
+    ones = np.ones(len(piv), np.uint32)
+    gpu_ctx.set_tuning(0, 64)
+    gpu_ctx.set_option("tune_log", 1)
+    try:
+        intra = cl.evaluate_resident()
+        cc = gpu_ctx.last_counters()
+        out = cl.pivot_phase(piv, ones, comm[piv], intra, np.zeros(n), prec=F.PREC_F64)
+        cp = gpu_ctx.last_counters()
+        gpu_ctx.set_option("autotune", 0)
+        ref = cl.pivot_phase(piv, ones, comm[piv], intra, np.zeros(n), prec=F.PREC_F64)
+    finally:
+        gpu_ctx.set_option("autotune", 1)
+        gpu_ctx.set_option("tune_log", 0)
+        gpu_ctx.set_tuning(0, 0)
+    err = capfd.readouterr().err
+    lines = [x for x in err.splitlines() if x.startswith("[fbc tune]")]
+    print("\n".join(lines), flush=True)
+    print(f"cluster delta {cc['delta']}, pivot delta {cp['delta']}", flush=True)
+    assert cc["tuned"] == 1 and cp["tuned"] == 1
+    assert any("mode 3: no region launch" in x and f"choice {cc['delta']})" in x for x in lines)
+    assert not any(x.startswith("[fbc tune] mode 3 region") for x in lines)
+    assert any(x.startswith("[fbc tune] mode 4 region") for x in lines)
+    scale = np.abs(ref).max()
+    assert np.abs(out - ref).max() <= 1e-12 * scale
+
+
 def test_autotune_cluster_eval_identical(gpu_ctx):
     """Cluster evaluation tuned (half-wavefront or wavefront teams, three
     deltas) equals the fixed configuration bit for bit: the intra-cluster
