@@ -410,15 +410,26 @@ __global__ __launch_bounds__(AWG) void k_km_msum(KmParams p, const uint32_t *til
     }
     __syncthreads();
     if (f >= D) return;
+    /* the features of MSB slots are loaded together before they are summed
+     * (an atomic between two loads would otherwise keep them apart); a slot
+     * past nm reads slot 0's row again and is not summed */
+    constexpr uint32_t MSB = 16;
     double acc = 0;
     uint32_t k = s_k[0];
-    for (uint32_t i = 0; i < nm; ++i) {
-        if (s_k[i] != k) {
-            unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
-            acc = 0;
-            k = s_k[i];
+    for (uint32_t i0 = 0; i0 < nm; i0 += MSB) {
+        int32_t xv[MSB];
+#pragma unroll
+        for (uint32_t j = 0; j < MSB; ++j) xv[j] = X[(uint64_t)s_v[i0 + j < nm ? i0 + j : 0] * D + f];
+#pragma unroll
+        for (uint32_t j = 0; j < MSB; ++j) {
+            if (i0 + j >= nm) break;
+            if (s_k[i0 + j] != k) {
+                unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
+                acc = 0;
+                k = s_k[i0 + j];
+            }
+            acc += (double)xv[j];
         }
-        acc += (double)X[(uint64_t)s_v[i] * D + f];
     }
     unsafeAtomicAdd(&msum[(uint64_t)k * D + f], acc);
 }
@@ -455,9 +466,22 @@ __global__ __launch_bounds__(AWG) void k_km_dist(KmParams p, const uint32_t *til
     for (uint32_t f0 = 0; f0 < D; f0 += SNAP_F) {
         const uint32_t f = f0 + lane;
         if (f < D) {
-#pragma unroll 4
-            for (uint32_t mi = wave; mi < nm; mi += AWG / 64)
-                s_d[mi][lane] = msum[(uint64_t)s_k[mi] * D + f] / s_n[mi] - (double)X[(uint64_t)s_v[mi] * D + f];
+            /* a wave's PW members: all their loads first (a member past nm
+             * reads member 0's again and is not stored) */
+            constexpr uint32_t PW = SNAP_M / (AWG / 64);
+            double mv[PW];
+            int32_t xv[PW];
+#pragma unroll
+            for (uint32_t j = 0; j < PW; ++j) {
+                const uint32_t mi = wave + j * (AWG / 64), mm = mi < nm ? mi : 0;
+                mv[j] = msum[(uint64_t)s_k[mm] * D + f];
+                xv[j] = X[(uint64_t)s_v[mm] * D + f];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < PW; ++j) {
+                const uint32_t mi = wave + j * (AWG / 64);
+                if (mi < nm) s_d[mi][lane] = mv[j] / s_n[mi] - (double)xv[j];
+            }
         }
         __syncthreads();
         if (threadIdx.x < nm) {
@@ -490,15 +514,26 @@ __global__ __launch_bounds__(AWG) void k_km_pick(KmParams p, const uint32_t *til
     const double *md = p.mdist + p.poff[c];
     uint32_t bv = cent[k];
     if (n) {
+        /* the smallest (distance, index) pair: a minimum, so the members'
+         * loads go PB at a time (a slot past n reads the first again) */
+        constexpr uint32_t PB = 8;
         bv = members[b0];
         double bd = md[b0];
-        for (uint32_t i = 1; i < n; ++i) {
-            const uint32_t v = members[b0 + i];
-            const double d = md[b0 + i];
-            if (d < bd || (d == bd && v < bv)) {
-                bv = v;
-                bd = d;
+        for (uint32_t i0 = 1; i0 < n; i0 += PB) {
+            uint32_t vv[PB];
+            double dd[PB];
+#pragma unroll
+            for (uint32_t j = 0; j < PB; ++j) {
+                const uint32_t i = i0 + j < n ? i0 + j : 0;
+                vv[j] = members[b0 + i];
+                dd[j] = md[b0 + i];
             }
+#pragma unroll
+            for (uint32_t j = 0; j < PB; ++j)
+                if (dd[j] < bd || (dd[j] == bd && vv[j] < bv)) {
+                    bv = vv[j];
+                    bd = dd[j];
+                }
         }
     }
     p.next[p.koff[c] + k] = bv;
