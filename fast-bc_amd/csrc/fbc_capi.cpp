@@ -68,6 +68,7 @@ struct fbc_ctx {
     uint32_t autotune = 1;        /* tune delta / team form on large batched runs */
     uint32_t ballot_append = 0;   /* wavefront-aggregated list appends */
     uint32_t tune_log = 0;        /* autotuner: a line per trial launch on stderr */
+    uint32_t km_budget_mb = 0;    /* k-means Gram budget in MiB (0: automatic; a test hook) */
     /* cluster evaluation on the LDS-view kernel: 0 never, 1 always, 2 when
      * the clusters whose rows fit in LDS carry at least half of the run's
      * work (lds_fit_share, set by the cluster evaluation for each launch) and
@@ -726,6 +727,7 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     else if (!std::strcmp(name, "ballot_append")) ctx->ballot_append = v ? 1 : 0;
     else if (!std::strcmp(name, "lds_view")) ctx->lds_view = std::min<uint32_t>(v, 2);
     else if (!std::strcmp(name, "tune_log")) ctx->tune_log = v ? 1 : 0;
+    else if (!std::strcmp(name, "kmeans_budget_mb")) ctx->km_budget_mb = v;
     else if (!std::strcmp(name, "class_key_bits"))
         ctx->class_key_mask = (v == 0 || v >= 64) ? ~0ull : ((1ull << v) - 1);
     else return fail(ctx, FBC_E_ARG, std::string("unknown option: ") + name);
@@ -1837,7 +1839,9 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
      * sets first: a group's initialisation lasts as long as its largest set's
      * sequential choice of centroids, so similar sizes share a group */
     uint64_t budget = std::min<uint64_t>(ctx->total_mem / 4, (uint64_t)64 << 30);
-    {
+    if (ctx->km_budget_mb) {
+        budget = (uint64_t)ctx->km_budget_mb << 20;
+    } else {
         /* and what is free now, less a sixteenth of the device */
         size_t freeb = 0, totalb = 0;
         if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
@@ -1846,6 +1850,13 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         }
         budget = std::max<uint64_t>(budget, 1);
     }
+    /* the slot memory of the Brandes launches is idle between them (every
+     * task initialises its own slot), so the Gram matrices use it when it is
+     * the larger: after a first computeBC it holds the pivot phase's slots
+     * (86 GB on the 1M-vertex grid, 129 GB on the 4M-vertex one, whose Gram
+     * matrices then need fewer groups) */
+    bool in_scratch = ctx->scratch && ctx->scratch_bytes > budget;
+    if (in_scratch) budget = ctx->scratch_bytes;
     std::vector<std::vector<uint32_t>> groups;
     uint64_t gmax = 0;
     {
@@ -1866,6 +1877,9 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
             gmax = std::max(gmax, used);
         }
     }
+    /* a set larger than the budget has a group of its own: then the largest
+     * group may not fit the slot memory, and gets its own allocation */
+    if (in_scratch && gmax * sizeof(double) > ctx->scratch_bytes) in_scratch = false;
     std::vector<uint32_t> P(npoints, npoints + nsets), D(nfeat, nfeat + nsets), K(k, k + nsets), active(nsets, 0);
     fbcdev::KmParams p;
     std::memset(&p, 0, sizeof(p));
@@ -1895,8 +1909,9 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
         (e = dalloc(&duk, ks)) != hipSuccess || (e = dalloc(&dnu, nsets)) != hipSuccess ||
         (e = dalloc(&dfirst, ps)) != hipSuccess ||
         (e = hipMemsetAsync(dfirst, 0xFF, ps * sizeof(uint32_t), ctx->stream)) != hipSuccess ||
-        (e = dalloc(&dG, gmax)) != hipSuccess)
+        (!in_scratch && (e = dalloc(&dG, gmax)) != hipSuccess))
         rc = hip_fail(ctx, e, "k-means setup");
+    if (!rc && in_scratch) dG = reinterpret_cast<double *>(ctx->scratch);
     if (!rc) {
         p.P = dP;
         p.D = dD;
@@ -2005,7 +2020,8 @@ static int kmeans_device(fbc_ctx *ctx, uint32_t nsets, const uint32_t *npoints, 
     }
     for (auto *q : {dP, dD, dK, dtiles, dcent, dnext, dcnt, dstart, dasg, dmem, dact, duk, dnu, dfirst}) dfree(q);
     for (auto *q : {dxo, dpo, dko, dmo, dgo}) dfree(q);
-    for (auto *q : {dxn, dcd, dmd, dmsum, dG, dkv}) dfree(q);
+    for (auto *q : {dxn, dcd, dmd, dmsum, dkv}) dfree(q);
+    if (!in_scratch) dfree(dG);
     dfree(dcw);
     ctx->last_ms = ms_total;
     ctx->last_te = 0;

@@ -163,7 +163,7 @@ class Context:
     def set_option(self, name, value):
         """Named tuning option (fbc_set_option): "delta", "slots", "frontier_sort",
         "relabel", "scratch_mb", "half_team_max", "class_key_bits", "autotune", "ballot_append",
-        "lds_view", "tune_log"."""
+        "lds_view", "tune_log", "kmeans_budget_mb"."""
         self.check(lib().fbc_set_option(self.h, name.encode(), int(value)))
 
     def last_stats(self):

@@ -103,8 +103,12 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
  * rows fit carry at least half of the run's work, n_c x m_c summed, and every
  * team gets 32 tasks or more), "tune_log" (1: the autotuner prints a line per
  * trial launch on stderr -- its delta, tasks, time and edges per ms, and each
- * region candidate's edges per busy tick; 0, the default: silent).  No
- * counterpart in the reference. */
+ * region candidate's edges per busy tick; 0, the default: silent),
+ * "kmeans_budget_mb" (device memory for the k-means Gram matrices, in MiB;
+ * sets share a Gram pass while theirs fit; 0, the default: 64 GiB or a
+ * quarter of the device, within what is free, or the slot memory of the
+ * Brandes launches, idle during the k-means, when that is larger; a test
+ * hook).  No counterpart in the reference. */
 int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value);
 
 /* Upload a graph: row_ptr[n+1] (uint64, row_ptr[n] = m), col[m], w[m].

@@ -79,6 +79,24 @@ def test_device_kmeans_matches_batch(gpu_ctx):
         assert np.array_equal(cw[kptr[c]:kptr[c + 1]], ow.astype(np.int32)), c
 
 
+def test_device_kmeans_gram_in_slot_memory(gpu_ctx):
+    """With a 1 MiB Gram budget the device k-means of cfg1's clusters puts its
+    Gram matrices in the slot memory the cluster evaluation left idle, and
+    chooses the same centroids and weights as with the automatic budget."""
+    print("device k-means cfg1, Gram matrices in the slot memory", flush=True)
+    g, cl = clusters_of(gpu_ctx, 1)
+    intra = cl.evaluate_resident()
+    cl.classes(intra)
+    ref = cl.kmeans(0.5)
+    gpu_ctx.set_option("kmeans_budget_mb", 1)
+    try:
+        got = cl.kmeans(0.5)
+    finally:
+        gpu_ctx.set_option("kmeans_budget_mb", 0)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
 def test_class_hash_collision_reported(gpu_ctx):
     print("class hash collisions cfg0", flush=True)
     g, cl = clusters_of(gpu_ctx, 0)

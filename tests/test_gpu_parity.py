@@ -237,6 +237,26 @@ def test_kmeans_batch_matches_reference(gpu_ctx):
         assert np.array_equal(cent, gold[f"cent{t}"]) and np.array_equal(cw, gold[f"cw{t}"]), t
 
 
+@pytest.mark.parametrize("fresh", [True, False])
+def test_kmeans_batch_small_gram_budget(gpu_ctx, fresh):
+    """A 1 MiB Gram budget: a context that has run no Brandes launch holds no
+    slot memory and runs its sets in groups of about one each; one that has
+    (the shared context, after the tests above) puts the Gram matrices in
+    its idle slot memory.  Either way the reference's centroids and weights."""
+    gold = np.load(os.path.join(GOLDEN, "kmeans_cases.npz"))
+    sets = [(int(gold[f"k{t}"][0]), interleave(gold[f"L{t}"], gold[f"C{t}"]), gold[f"W{t}"]) for t in range(24)]
+    ctx = F.Context(0) if fresh else gpu_ctx
+    ctx.set_option("kmeans_budget_mb", 1)
+    try:
+        got = ctx.kmeans_batch(sets)
+    finally:
+        ctx.set_option("kmeans_budget_mb", 0)
+        if fresh:
+            ctx.close()
+    for t, (cent, cw) in enumerate(got):
+        assert np.array_equal(cent, gold[f"cent{t}"]) and np.array_equal(cw, gold[f"cw{t}"]), t
+
+
 def test_kmeans_batch_matches_oracle(gpu_ctx):
     rng = np.random.default_rng(77)
     sets, expect = [], []
