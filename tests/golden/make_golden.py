@@ -33,6 +33,9 @@ the default `--part base` writes the ones above):
   cfg4_reference.npz    on the 2048x2048 road grid: digest and community
                         count of the reference Louvain partition; dependency
                         digests of two sources
+  degenerate.npz        (`--part degenerate`) five small graphs of unusual
+                        shape: partitions, exact BC, intra-cluster BC and
+                        weighted dependency sums from the reference's code
 """
 import hashlib
 import json
@@ -171,6 +174,81 @@ def ref_dep_rows(ref, graph, sources, tmp):
     return [np.loadtxt(op, comments="#") for op in outs]
 
 
+def degenerate_cases():
+    """Small graphs with the shapes the benchmark grids lack: a 2-cycle, ids
+    with no arcs, two components, a star, and a random graph with self-loops,
+    repeated arcs (the loader sums them) and vertices no arc reaches."""
+    yield "two_cycle", [0, 1], [1, 0], [3, 5]
+    yield "isolated_ids", [0, 5, 9], [5, 9, 0], [1, 1, 1]
+    yield "two_components", [0, 1, 2, 3, 0, 4, 5, 6, 7, 4], [1, 2, 3, 0, 2, 5, 6, 7, 4, 6], [2, 2, 2, 2, 3, 1, 4, 1, 4, 5]
+    hub = 20
+    yield "star", [0] * hub + list(range(1, hub + 1)), list(range(1, hub + 1)) + [0] * hub, [1] * (2 * hub)
+    rng = np.random.default_rng(7)
+    n = 60
+    s = [int(x) for x in rng.integers(0, n, 220)]
+    d = [int(x) for x in rng.integers(0, n, 220)]
+    w = [int(x) for x in rng.integers(1, 8, 220)]
+    s.append(n - 1), d.append(0), w.append(3)
+    for v in rng.choice(n, 8, replace=False):
+        s.append(int(v)), d.append(int(v)), w.append(int(rng.integers(1, 8)))
+    for i in rng.choice(len(s), 20, replace=False):
+        s.append(s[i]), d.append(d[i]), w.append(int(rng.integers(1, 8)))
+    yield "loops_duplicates", s, d, w
+
+
+def part_degenerate(tmp):
+    """degenerate.npz: per case, the arcs; the reference Louvain partition
+    (seeds 1,2,3,4, precision 0.01); ExactBrandesBC (1 thread); the intra-
+    cluster BC of DijkstraClusterEvaluator over every bordered community (the
+    reference build skips borderless ones, main.cpp:6); and, for kfrac 0 and
+    0.5, the oracle's pivots and weights (the pivot selectors include spdlog
+    and do not build here) with the reference's DijkstraSSBrandesBC sum over
+    them (ssb mode, 1 thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    res = {}
+    for name, s, d, w in degenerate_cases():
+        s, d, w = (np.asarray(a, np.int32) for a in (s, d, w))
+        gp = os.path.join(tmp, f"{name}.txt")
+        with open(gp, "w") as f:
+            f.writelines(f"{a} {b} {c}\n" for a, b, c in zip(s, d, w))
+        res[f"{name}_src"], res[f"{name}_dst"], res[f"{name}_w"] = s, d, w
+        comm = np.array(run([REF, "louvain", gp, "0.01", "1", "2", "3", "4"], env={"OMP_NUM_THREADS": "4"}).split(),
+                        dtype=np.int32)
+        res[f"{name}_comm"] = comm
+        ep = os.path.join(tmp, f"{name}_exact.txt")
+        run([REF, "exact", gp, ep])
+        res[f"{name}_exact"] = np.loadtxt(ep, ndmin=1)
+        intra = np.zeros(len(comm))
+        for c in np.unique(comm):
+            verts = np.where(comm == c)[0]
+            vp = os.path.join(tmp, f"{name}_c{c}.txt")
+            np.savetxt(vp, verts, fmt="%d")
+            op = os.path.join(tmp, f"{name}_c{c}_out.txt")
+            run([REF, "cluster", gp, vp, op])
+            lines = open(op).read().split("\n")
+            if len(lines[0].split()) > 1:   # bordered
+                rows = [ln.split() for ln in lines[1:] if ln]
+                intra[verts] = [float(r[1]) for r in rows]
+        res[f"{name}_intra"] = intra
+        og = O.Graph.from_file(gp)
+        for tag, kf in (("k0", 0.0), ("k5", 0.5)):
+            r = og.clustered_bc([1, 2, 3, 4], 0.01, kfrac=kf)
+            assert np.array_equal(r["comm"], comm)
+            piv, wts = r["pivots"].astype(np.int32), r["weights"].astype(np.int32)
+            res[f"{name}_{tag}_pivots"], res[f"{name}_{tag}_weights"] = piv, wts
+            res[f"{name}_{tag}_pcl"] = r["pivot_cluster"].astype(np.int32)
+            ssb = np.zeros(len(comm))
+            if len(piv):
+                pp = os.path.join(tmp, f"{name}_{tag}_piv.txt")
+                np.savetxt(pp, np.stack([piv, wts], 1), fmt="%d")
+                so = os.path.join(tmp, f"{name}_{tag}_ssb.txt")
+                run([REF, "ssb", gp, pp, so])
+                ssb = np.loadtxt(so, comments="#", ndmin=1)
+            res[f"{name}_{tag}_ssb"] = ssb
+    np.savez_compressed(os.path.join(HERE, "degenerate.npz"), **res)
+
+
 def part_cfg3(tmp):
     n, arcs = F.generate_config(3)
     p = os.path.join(tmp, "cfg3.txt")
@@ -287,7 +365,7 @@ def part_cfg4(tmp):
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
-    ap.add_argument("--part", default="base", choices=["base", "cfg3", "cfg2", "cfg2_oracle", "cfg4"])
+    ap.add_argument("--part", default="base", choices=["base", "cfg3", "cfg2", "cfg2_oracle", "cfg4", "degenerate"])
     a = ap.parse_args()
     if a.part == "base":
         main()
@@ -295,5 +373,6 @@ if __name__ == "__main__":
         if not os.path.exists(REF):
             sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref, needs /root/reference)")
         t = tempfile.mkdtemp(prefix="fbc_golden_")
-        {"cfg3": part_cfg3, "cfg2": part_cfg2, "cfg2_oracle": part_cfg2_oracle, "cfg4": part_cfg4}[a.part](t)
+        {"cfg3": part_cfg3, "cfg2": part_cfg2, "cfg2_oracle": part_cfg2_oracle, "cfg4": part_cfg4,
+         "degenerate": part_degenerate}[a.part](t)
         print("written", a.part)
