@@ -246,6 +246,39 @@ def part_degenerate(tmp):
                 run([REF, "ssb", gp, pp, so])
                 ssb = np.loadtxt(so, comments="#", ndmin=1)
             res[f"{name}_{tag}_ssb"] = ssb
+    # VertexInfo counts beyond the int range: a 20x20 unit-weight grid (each
+    # vertex to its four neighbours) as one community, a 4-cycle tied to its
+    # far corner as another.  sigma from a corner to the border (the far
+    # corner) is C(38,19) > INT_MAX, which DijkstraClusterEvaluator's (int)
+    # conversion (DijkstraClusterEvaluator.h:172-175) turns into INT_MIN on
+    # x86-64.
+    R = 20
+    s, d = [], []
+    for r in range(R):
+        for c in range(R):
+            for dr, dc in ((0, 1), (1, 0), (0, -1), (-1, 0)):
+                if 0 <= r + dr < R and 0 <= c + dc < R:
+                    s.append(r * R + c), d.append((r + dr) * R + c + dc)
+    far, n0 = R * R - 1, R * R
+    for a, b in ((far, n0), (n0, far), (n0, n0 + 1), (n0 + 1, n0 + 2), (n0 + 2, n0 + 3), (n0 + 3, n0),
+                 (n0 + 1, n0), (n0 + 2, n0 + 1), (n0 + 3, n0 + 2), (n0, n0 + 3)):
+        s.append(a), d.append(b)
+    gp = os.path.join(tmp, "bigsig.txt")
+    with open(gp, "w") as f:
+        f.writelines(f"{a} {b} 1\n" for a, b in zip(s, d))
+    vp = os.path.join(tmp, "bigsig_verts.txt")
+    np.savetxt(vp, np.arange(n0), fmt="%d")
+    op = os.path.join(tmp, "bigsig_out.txt")
+    run([REF, "cluster", gp, vp, op])
+    lines = open(op).read().split("\n")
+    rows = [ln.split() for ln in lines[1:] if ln]
+    res["bigsig_src"], res["bigsig_dst"] = np.asarray(s, np.int32), np.asarray(d, np.int32)
+    res["bigsig_part"] = np.repeat(np.int32([0, 1]), [n0, 4])
+    res["bigsig_borders"] = np.array(lines[0].split()[1:], dtype=np.int32)
+    res["bigsig_intra"] = np.array([float(r[1]) for r in rows])
+    res["bigsig_len"] = np.array([[float(x) for x in r[2::2]] for r in rows])
+    res["bigsig_cnt"] = np.array([[int(x) for x in r[3::2]] for r in rows], dtype=np.int32)
+    assert (res["bigsig_cnt"] == np.iinfo(np.int32).min).any()
     np.savez_compressed(os.path.join(HERE, "degenerate.npz"), **res)
 
 

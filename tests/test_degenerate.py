@@ -128,3 +128,53 @@ def test_gpu_clustered(name, tag, kfrac, gpu_ctx):
         assert np.array_equal(out["pivots"], Z[f"{name}_{tag}_pivots"].astype(np.uint32))
         assert np.array_equal(out["weights"], Z[f"{name}_{tag}_weights"].astype(np.uint32))
         close(out["bc"], expect, rtol, 1e-12 * mag.max())
+
+
+# ---- VertexInfo counts beyond the int range ----------------------------------
+# A 20x20 unit-weight grid community whose corner-to-corner path count,
+# C(38,19), exceeds INT_MAX: the reference's (int) conversion of it
+# (DijkstraClusterEvaluator.h:172-175) gives INT_MIN on x86-64, and the pivot
+# classes and k-means points are built from those counts.
+
+BIG_PART = Z["bigsig_part"]
+
+
+def bigsig_arcs():
+    s, d = Z["bigsig_src"], Z["bigsig_dst"]
+    return s, d, np.ones(len(s), np.int32)
+
+
+def test_oracle_big_sigma_counts():
+    s, d, w = bigsig_arcs()
+    og = O.Graph.from_arcs(s, d, w.astype(np.float64))
+    verts = np.where(BIG_PART == 0)[0]
+    bc, b, L, C = og.cluster_eval(verts)
+    assert np.array_equal(b, Z["bigsig_borders"])
+    assert np.array_equal(bc[verts], Z["bigsig_intra"])
+    assert np.array_equal(L, Z["bigsig_len"])
+    assert np.array_equal(C, Z["bigsig_cnt"])
+    assert (C == np.iinfo(np.int32).min).sum() == 13
+
+
+@pytest.mark.gpu
+def test_gpu_big_sigma_counts(gpu_ctx):
+    s, d, w = bigsig_arcs()
+    og = O.Graph.from_arcs(s, d, w.astype(np.float64))
+    rp, col, ww = og.csr()
+    g = F.Graph(gpu_ctx, og.n, rp, col, ww.astype(np.uint32))
+    cl = g.clusters(comm_of=BIG_PART)
+    bc, L, C = cl.evaluate()
+    assert np.array_equal(cl.borders(0), Z["bigsig_borders"].astype(np.uint32))
+    assert np.array_equal(cl.vi_rows(L, 0).astype(np.float64), Z["bigsig_len"])
+    assert np.array_equal(cl.vi_rows(C, 0), Z["bigsig_cnt"])
+    assert np.array_equal(bc[cl.members(0)], Z["bigsig_intra"])
+    for kfrac in (0.0, 0.5):
+        ref = O.clustered_bc_partition(og, BIG_PART, kfrac=kfrac)
+        mag = np.abs(ref["intra"]).max() * (1 + len(ref["pivots"]))
+        for prec, form, rtol in ((F.PREC_F64, F.FORM_BATCHED, 1e-9), (F.PREC_F32C, F.FORM_BATCHED, 1e-6),
+                                 (F.PREC_F64, F.FORM_PLAIN, 1e-9)):
+            out = F.clustered_bc(s, d, w, [1], kfrac=kfrac, prec=prec, form=form, partition=BIG_PART)
+            assert np.array_equal(out["intra"], ref["intra"])
+            assert np.array_equal(out["pivots"], ref["pivots"].astype(np.uint32))
+            assert np.array_equal(out["weights"], ref["weights"].astype(np.uint32))
+            close(out["bc"], ref["bc"], rtol, 1e-12 * mag)
