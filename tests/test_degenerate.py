@@ -12,6 +12,7 @@ BC bit for bit, the oracle's pivots (the reference's selectors need spdlog
 and are not built here), and BC within 1e-9 relative of the composition
 assembled from the reference's pieces (ClusteredBrandesBC.h:130-166)."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -178,3 +179,30 @@ def test_gpu_big_sigma_counts(gpu_ctx):
             assert np.array_equal(out["pivots"], ref["pivots"].astype(np.uint32))
             assert np.array_equal(out["weights"], ref["weights"].astype(np.uint32))
             close(out["bc"], ref["bc"], rtol, 1e-12 * mag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["exact", "2-clustered"])
+def test_cli_degenerate_text(tmp_path, mode):
+    """The fbc command line on the text of the random graph: the loader sums
+    repeated arcs and counts each distinct arc once (the log line), and
+    bc.txt writes the negative clustered values as 0 (main.cpp:282-294)."""
+    s, d, w = arcs("loops_duplicates")
+    g = str(tmp_path / "g.txt")
+    with open(g, "w") as f:
+        f.writelines(f"{a} {b} {c}\n" for a, b, c in zip(s, d, w))
+    out, full = str(tmp_path / "bc.txt"), str(tmp_path / "bc_full.txt")
+    args = ["--exact"] if mode == "exact" else ["-s", "1,2,3,4", "-e", "4", "-p", "0.01", "-k", "0.5"]
+    r = subprocess.run([F.FBC_BIN, *args, "-o", out, "--full-output", full, g], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    og = oracle_graph("loops_duplicates")
+    assert f"Loaded graph contains 60 vertices and {og.edges} edges" in r.stdout
+    vals = np.loadtxt(full)
+    if mode == "exact":
+        close(vals, Z["loops_duplicates_exact"], 1e-12)
+    else:
+        expect, mag = expected_clustered("loops_duplicates", "k5")
+        assert (expect < 0).any()
+        close(vals, expect, 1e-9, 1e-12 * mag.max())
+    assert open(out).read() == "".join(("%g" % x if x >= 0 else "0") + "\n" for x in vals)
