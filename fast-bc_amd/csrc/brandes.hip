@@ -295,7 +295,8 @@ struct Shared {
     Ctl ctl;
     Local loc;
     uint32_t task;
-    uint32_t sortbuf[SORT_CAP_MAX];
+    /* the counting sort's words: the entries, the buckets, the waves' sums */
+    uint32_t sortbuf[SORT_CHUNK_MAX + SORT_BUCKETS + WG / 64];
     uint32_t lq[2 * LQ_WG];
 };
 
@@ -313,30 +314,59 @@ struct SharedHalf {
     uint32_t lq[HALVES][2 * LQ_HALF];
 };
 
-/* Sort q[0, n) ascending through LDS (bitonic; n <= SORT_CAP_MAX), so that a
- * round's or a level's vertices are expanded in id order: neighbouring ids
- * share cache lines of the per-vertex state and of the CSR. */
-__device__ void sort_vertices(uint32_t *q, uint32_t n, uint32_t *buf)
+/* Bucket shift of a view of n vertices: the smallest that maps every id
+ * into SORT_BUCKETS buckets. */
+__device__ __forceinline__ uint32_t sort_shift(uint32_t n)
 {
-    uint32_t p2 = 64;
-    while (p2 < n) p2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < p2; i += WG) buf[i] = i < n ? q[i] : 0xFFFFFFFFu;
+    uint32_t sh = 0;
+    while (sh < 32 && ((n - 1) >> sh) >= SORT_BUCKETS) ++sh;
+    return sh;
+}
+
+/* Group q[0, n) by v >> shift (n <= SORT_CHUNK_MAX; ascending buckets, any
+ * order within one) with one counting pass through LDS, so that a round's or
+ * a level's vertices are taken in id order at bucket granularity:
+ * neighbouring ids share cache lines of the per-vertex state and of the CSR,
+ * and a wavefront's loads then fall on fewer lines.  buf holds
+ * SORT_CHUNK_MAX entries, SORT_BUCKETS counters and WG/64 wave sums.  Whole
+ * workgroup; ends with a barrier. */
+__device__ __noinline__ void bucket_vertices(uint32_t *q, uint32_t n, uint32_t shift, uint32_t *buf)
+{
+    constexpr uint32_t PER = SORT_BUCKETS / WG;
+    static_assert(SORT_BUCKETS % WG == 0 && WG % 64 == 0, "bucket scan layout");
+    uint32_t *const hist = buf + SORT_CHUNK_MAX, *const wsum = hist + SORT_BUCKETS;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    for (uint32_t i = tid; i < SORT_BUCKETS; i += WG) hist[i] = 0;
     __syncthreads();
-    for (uint32_t k = 2; k <= p2; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < p2; i += WG) {
-                const uint32_t o = i ^ j;
-                if (o > i) {
-                    const uint32_t a = buf[i], b = buf[o];
-                    if ((a > b) == ((i & k) == 0)) {
-                        buf[i] = b;
-                        buf[o] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    for (uint32_t i = threadIdx.x; i < n; i += WG) q[i] = buf[i];
+    for (uint32_t i = tid; i < n; i += WG) {
+        const uint32_t v = q[i];
+        buf[i] = v;
+        atomicAdd(&hist[min(v >> shift, SORT_BUCKETS - 1)], 1u);
+    }
+    __syncthreads();
+    /* exclusive prefix sums of the buckets: PER consecutive ones per thread,
+     * a wavefront scan of the threads' totals, then the waves' totals */
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < PER; ++k) tot += hist[tid * PER + k];
+    uint32_t inc = tot;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[tid >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - tot;
+    for (uint32_t w = 0; w < (tid >> 6); ++w) off += wsum[w];
+    for (uint32_t k = 0; k < PER; ++k) {   /* each thread rewrites only its own counters */
+        const uint32_t c = hist[tid * PER + k];
+        hist[tid * PER + k] = off;
+        off += c;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += WG) {
+        const uint32_t v = buf[i];
+        q[atomicAdd(&hist[min(v >> shift, SORT_BUCKETS - 1)], 1u)] = v;
+    }
     __syncthreads();
 }
 
@@ -662,7 +692,14 @@ __device__ __forceinline__ void phase_distances(const View &V, const Slot S, con
             }
             if (t.lleader()) L.nheavy[(rnd + 1) & 1] = 0;
             if constexpr (IsWG<T>::value)
-                if (near > 64 && near <= sort_cap) sort_vertices(near <= T::LQ ? lqa : qa, near, t.sortbuf);
+                if (near > 64 && sort_cap) {
+                    /* the queue's LDS head, then its remainder in HBM */
+                    const uint32_t sh = sort_shift(V.n), head = min(near, T::LQ);
+                    for (uint32_t c0 = 0; c0 < head; c0 += sort_cap)
+                        bucket_vertices(lqa + c0, min(sort_cap, head - c0), sh, t.sortbuf);
+                    for (uint32_t c0 = T::LQ; c0 < near; c0 += sort_cap)
+                        bucket_vertices(qa + c0, min(sort_cap, near - c0), sh, t.sortbuf);
+                }
             for (uint32_t i = r0; i < near; i += R) {
                 const uint32_t u = near_entry<T>(lqa, qa, i);
                 ++ct.c[STAT_VISITS];
@@ -1006,8 +1043,10 @@ __device__ __forceinline__ uint32_t sigma_form(const View &V, const Slot S, cons
         if (t.leader()) ast32<SC>(&c.lcnt[(lvl + 2) % 3], 0u);
         bool sorted = false;   /* a sorted level is read back from S.qa */
         if constexpr (IsWG<T>::value)
-            if (le - lb > 64 && le - lb <= sort_cap) {
-                sort_vertices(S.qa + lb, le - lb, t.sortbuf);
+            if (le - lb > 64 && sort_cap) {   /* every entry of a level is in S.qa */
+                const uint32_t sh = sort_shift(V.n);
+                for (uint32_t c0 = lb; c0 < le; c0 += sort_cap)
+                    bucket_vertices(S.qa + c0, min(sort_cap, le - c0), sh, t.sortbuf);
                 sorted = true;
             }
         for (uint32_t i = lb + r0; i < le; i += R) {
@@ -1408,7 +1447,7 @@ __device__ __forceinline__ void run_task(const BrandesParams &p, const Slot S, c
     const uint64_t t0 = wall_clock64();
     phase_distances(V, S, t, src, delta, p.sort_cap, p.err, ct);
     const uint64_t t1 = wall_clock64();
-    const uint32_t nlev = phase_sigma(V, S, t, src, p.sort_cap, p.err, ct);
+    const uint32_t nlev = phase_sigma(V, S, t, src, p.level_sort, p.err, ct);
     const uint64_t t2 = wall_clock64();
     if (t.leader()) {
         ct.c[STAT_T_DIST] += t1 - t0;

@@ -45,7 +45,8 @@ struct fbc_ctx {
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t half_max = fbcdev::HALF_TEAM_MAX_N;   /* views up to this many vertices: half-wavefront teams */
-    uint32_t sort_cap = 0;            /* batched kernel: frontier / level sort limit */
+    uint32_t sort_cap = fbcdev::SORT_CHUNK_MAX;     /* workgroup teams: near-queue sort chunk (0: off) */
+    uint32_t level_sort = fbcdev::SORT_CHUNK_MAX;   /* workgroup teams: DAG-level sort chunk (0: off) */
     uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
     uint64_t class_key_mask = ~0ull;  /* class hashes kept (fewer bits: collisions, a test hook) */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */
@@ -550,6 +551,7 @@ int run(fbc_ctx *ctx, const fbc_graph *cg, int mode, int prec, bool ordered, Bra
     ctx->last_sigma_inexact = 0;
     if (p.ntasks == 0) return 0;
     p.sort_cap = ctx->sort_cap;
+    p.level_sort = ctx->level_sort;
     p.ballot_append = ctx->ballot_append;
     uint32_t err = 0;
     int rc = 0;
@@ -719,7 +721,8 @@ int fbc_set_option(fbc_ctx *ctx, const char *name, int64_t value)
     const uint32_t v = (uint32_t)value;
     if (!std::strcmp(name, "delta")) ctx->delta_override = v;
     else if (!std::strcmp(name, "slots")) ctx->slots_override = v;
-    else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CAP_MAX);
+    else if (!std::strcmp(name, "frontier_sort")) ctx->sort_cap = std::min(v, fbcdev::SORT_CHUNK_MAX);
+    else if (!std::strcmp(name, "level_sort")) ctx->level_sort = std::min(v, fbcdev::SORT_CHUNK_MAX);
     else if (!std::strcmp(name, "relabel")) ctx->relabel = v ? 1 : 0;
     else if (!std::strcmp(name, "scratch_mb")) ctx->scratch_mb = v;
     else if (!std::strcmp(name, "half_team_max")) ctx->half_max = std::min<uint32_t>(v, fbcdev::WAVE_TEAM_MAX_N);
@@ -1085,6 +1088,7 @@ int fbc_dep_batch_noalloc(fbc_ctx *ctx, const fbc_graph *g, const uint32_t *d_sr
     p.half_max = ctx->half_max;
     p.delta = auto_delta(ctx, g);
     p.sort_cap = ctx->sort_cap;
+    p.level_sort = ctx->level_sort;
     p.ballot_append = ctx->ballot_append;
     p.queue_head = reinterpret_cast<uint32_t *>(base);
     p.err = reinterpret_cast<uint32_t *>(base + 4);

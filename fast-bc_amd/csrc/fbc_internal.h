@@ -92,7 +92,8 @@ struct BrandesParams {
     unsigned long long *te;   /* traversed-edge counter */
     unsigned long long *stats;/* STAT_COUNT work counters, summed over tasks */
     uint32_t *grid_ctl;       /* plain kernel: traversal counters shared by the grid (16 words) */
-    uint32_t sort_cap;        /* batched kernel: sort near queues / levels of up to this many vertices (0: off) */
+    uint32_t sort_cap;        /* workgroup teams: group near queues by id, in chunks of this many entries (0: off) */
+    uint32_t level_sort;      /* workgroup teams: the same for DAG levels (0: off) */
     uint32_t nslots;          /* slots the scratch holds (the wavefront kernels check their slot against it) */
     uint32_t half_max;        /* views up to this many vertices: half-wavefront teams */
     uint32_t ballot_append;   /* 1: queue and level appends take one atomic per wavefront */
@@ -101,8 +102,13 @@ struct BrandesParams {
     uint32_t lds_rows_max;
 };
 
-/* Largest near queue or level the batched kernel sorts (LDS words). */
-constexpr uint32_t SORT_CAP_MAX = 1024;
+/* Workgroup teams group a round's near queue and each DAG level by vertex
+ * id before expanding them, with a one-pass counting sort in LDS: ids go to
+ * SORT_BUCKETS buckets by v >> shift (the smallest shift that maps the view
+ * into them), at most SORT_CHUNK_MAX entries at a time (longer lists are
+ * grouped chunk by chunk). */
+constexpr uint32_t SORT_CHUNK_MAX = 4096;
+constexpr uint32_t SORT_BUCKETS = 2048;
 
 /* Work counters of the persistent kernel (fbc_last_stats_ex). */
 enum Stat : int {

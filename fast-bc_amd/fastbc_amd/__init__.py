@@ -161,7 +161,7 @@ class Context:
         self.check(lib().fbc_set_tuning(self.h, bucket_delta, slots))
 
     def set_option(self, name, value):
-        """Named tuning option (fbc_set_option): "delta", "slots", "frontier_sort",
+        """Named tuning option (fbc_set_option): "delta", "slots", "frontier_sort", "level_sort",
         "relabel", "scratch_mb", "half_team_max", "class_key_bits", "autotune", "ballot_append",
         "lds_view", "tune_log", "kmeans_budget_mb"."""
         self.check(lib().fbc_set_option(self.h, name.encode(), int(value)))

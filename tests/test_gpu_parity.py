@@ -104,19 +104,29 @@ def test_cfg0_exact_bc(cfg0):
         assert (np.abs(bc - r2) / np.maximum(np.abs(r2), 1e-12 * np.abs(r2).max())).max() <= 1e-9
 
 
-def test_cfg0_sorted_frontier_bitexact(cfg0, gpu_ctx):
-    """Sorting the near queues and DAG levels changes only the order in which
-    vertices are expanded: distances, sigma and FP64 dependencies stay exact."""
-    g, og = cfg0
-    srcs = np.random.default_rng(5).choice(g.n, 16, replace=False)
-    gpu_ctx.set_option("frontier_sort", 1024)
+def test_sorted_frontier_bitexact(gpu_ctx):
+    """Grouping the near queues and DAG levels by vertex id (workgroup teams:
+    views of more than 65,536 vertices) changes only the order in which
+    vertices are expanded: distances, sigma and FP64 dependencies stay
+    exact with the grouping off, on (the default, chunks of 4,096 entries)
+    and in chunks of 100 (a round's or level's list grouped piece by piece,
+    the LDS head and the HBM remainder of a near queue separately)."""
+    n, (s, d, w) = F.generate_grid(300, 300, 1000, 7, road=True)
+    assert n > 65536
+    g, og = both_graphs(gpu_ctx, n, s, d, w)
+    srcs = np.random.default_rng(5).choice(n, 6, replace=False)
+    ref = [og.single_source_dep(int(x)) for x in srcs]
     try:
-        check_sssp(g, og, srcs[:8])
-        dep = g.dep_batch(srcs)
+        for chunk in (0, 100, 4096):
+            gpu_ctx.set_option("frontier_sort", chunk)
+            gpu_ctx.set_option("level_sort", chunk)
+            check_sssp(g, og, srcs[:3])
+            dep = g.dep_batch(srcs)
+            for i, x in enumerate(srcs):
+                assert np.array_equal(dep[i], ref[i]), f"chunk {chunk}, source {x}"
     finally:
-        gpu_ctx.set_option("frontier_sort", 0)
-    for i, s in enumerate(srcs):
-        assert np.array_equal(dep[i], og.single_source_dep(int(s))), f"source {s}"
+        gpu_ctx.set_option("frontier_sort", 4096)
+        gpu_ctx.set_option("level_sort", 4096)
 
 
 def test_cfg0_dep_f32c(cfg0):
