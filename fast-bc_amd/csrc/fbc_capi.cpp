@@ -45,8 +45,8 @@ struct fbc_ctx {
     std::string err;
     uint32_t delta_override = 0, slots_override = 0;
     uint32_t half_max = fbcdev::HALF_TEAM_MAX_N;   /* views up to this many vertices: half-wavefront teams */
-    uint32_t sort_cap = fbcdev::SORT_CHUNK_MAX;     /* workgroup teams: near-queue sort chunk (0: off) */
-    uint32_t level_sort = fbcdev::SORT_CHUNK_MAX;   /* workgroup teams: DAG-level sort chunk (0: off) */
+    uint32_t sort_cap = 0;     /* workgroup teams: near-queue sort chunk (0: off, the default) */
+    uint32_t level_sort = 0;   /* workgroup teams: DAG-level sort chunk (0: off, the default) */
     uint32_t scratch_mb = 0;          /* slot memory limit in MiB (0: automatic) */
     uint64_t class_key_mask = ~0ull;  /* class hashes kept (fewer bits: collisions, a test hook) */
     uint32_t relabel = 1;             /* pivot phase / exact on the locality-relabelled graph */

@@ -72,8 +72,8 @@ int fbc_set_tuning(fbc_ctx *ctx, uint32_t bucket_delta, uint32_t slots);
 /* Named tuning option (0 = automatic / off): "delta", "slots" (as above),
  * "frontier_sort" (workgroup teams, i.e. views of more than 65,536
  * vertices, group each round's near queue by vertex-id range before
- * expanding it, in chunks of this many entries, up to and by default 4096;
- * 0: off), "level_sort" (the same for the shortest-path DAG's levels, which
+ * expanding it, in chunks of this many entries, up to 4096; 0, the default:
+ * off), "level_sort" (the same for the shortest-path DAG's levels, which
  * the dependency phase then reads in that order too), "relabel"
  * (1, the default: the batched pivot phase and fbc_exact_bc run on a copy of
  * the graph renumbered into BFS-grown blocks of 16 vertices, for cache-line
